@@ -1,0 +1,268 @@
+"""Benchmark: batched multi-orbital 3-D B-spline evaluation on one MI355X.
+
+Default workload (the task's headline configuration): a periodic 48^3 FP32
+multi_UBspline_3d_s table with 256 orbitals, each a sum of 8 random plane
+waves (integer k, |k_i| <= 4, N(0,1) amplitudes), coefficients from this
+library's create/set; vgh at 32768 uniform random points (256 walkers x 128
+electrons) in one launch.  One step = one einspline_eval_vgh_batch launch
+over all points, inputs resident in HBM.
+
+Printed: one JSON line with evals/s (points x orbitals / s), the roofline
+of the kernel (algorithmic bytes per launch / measured launch time vs the
+8 TB/s HBM peak), the CPU baseline (the oracle's restatement of the
+reference's std evaluation, OpenMP over points on the host cores) and, with
+--precision, the FP32 / BF16 / FP16 table comparison on the 96^3, 512-orbital
+workload with errors against the FP64 evaluation.
+
+    python bench.py [--steps K] [--warmup W] [--precision] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import einspline_amd as ea  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "orbital-vgh evals/sec + %HBM roofline, FP32 vs BF16 table; serving p99 latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# synthetic orbitals
+
+
+def plane_wave_block(rng, M, count, nwaves=8, kmax=4):
+    """count orbitals sampled on the periodic M^3 grid r = i/M, each
+    f(r) = sum_m a_m cos(2 pi k_m.r) + b_m sin(2 pi k_m.r)."""
+    x = np.arange(M) / M
+    out = np.empty((count, M, M, M), np.float64)
+    for o in range(count):
+        K = rng.integers(-kmax, kmax + 1, size=(nwaves, 3))
+        a = rng.standard_normal(nwaves)
+        b = rng.standard_normal(nwaves)
+        acc = np.zeros((M, M, M), np.complex128)
+        for m in range(nwaves):
+            ex = np.exp(2j * np.pi * K[m, 0] * x)
+            ey = np.exp(2j * np.pi * K[m, 1] * x)
+            ez = np.exp(2j * np.pi * K[m, 2] * x)
+            acc += (a[m] - 1j * b[m]) * (ex[:, None, None] * ey[None, :, None] * ez[None, None, :])
+        out[o] = acc.real
+    return out
+
+
+def build_table(kind, M, N, seed, block=32):
+    rng = np.random.default_rng(seed)
+    h = ea.HostSpline(kind, (M, M, M), (0, 0, 0), (1, 1, 1), N)
+    for first in range(0, N, block):
+        cnt = min(block, N - first)
+        h.set_block(first, plane_wave_block(rng, M, cnt))
+    return h
+
+
+# ---------------------------------------------------------------------------
+# timing
+
+
+def time_batches(tab, mode, pos, P, outs, steps, warmup, packed=False):
+    stream = ea.Stream()
+    for _ in range(warmup):
+        tab.eval_batch(mode, pos, P, outs, packed, stream)
+    stream.synchronize()
+    ea.synchronize()
+    e0, e1 = ea.Event(), ea.Event()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        tab.eval_batch(mode, pos, P, outs, packed, stream)
+    e1.record(stream)
+    e1.synchronize()
+    ea.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    kern = e0.elapsed_ms(e1) / steps
+    return wall, kern
+
+
+def bytes_per_eval(coef_bytes, mode, packed, out_bytes):
+    nout = {"v": 1, "vg": 4, "vgl": 5, "vgh": 10 if packed else 13}[mode]
+    return 64 * coef_bytes + nout * out_bytes
+
+
+def cpu_baseline(kind, host, pts, mode, target_s=10.0):
+    """The oracle (restatement of the reference's std eval) on the host cores,
+    on a bounded sample of the same workload."""
+    import oracle
+
+    N = host.N
+    M = host.M
+    o = oracle.Table(kind, M, (0, 0, 0), (1, 1, 1), N)
+    o.coefs[...] = host.coefs().reshape(o.coefs.shape)
+    threads = min(os.cpu_count() or 1, 16)
+    n = 128
+    t0 = time.perf_counter()
+    o.eval_batch(mode, pts[:n], nthreads=threads)
+    dt = time.perf_counter() - t0
+    n2 = int(min(len(pts), max(n, n * target_s / max(dt, 1e-6) / 3)))
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.eval_batch(mode, pts[:n2], nthreads=threads)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {"value": n2 * N / best, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n2} of the {len(pts)} points x {N} orbitals, {mode}, best of 3 runs"}
+
+
+def load_traffic(name):
+    f = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(f):
+        with open(f) as fh:
+            return json.load(fh)
+    return None
+
+
+# ---------------------------------------------------------------------------
+# workloads
+
+
+def headline(args):
+    M, N, P = 48, 256, 32768
+    t0 = time.perf_counter()
+    host = build_table("s", M, N, seed=2024)
+    log(f"[bench] built 48^3 x 256 FP32 table in {time.perf_counter() - t0:.1f}s")
+    tab = ea.DeviceTable(host)
+    if args.lane_bytes or args.block:
+        tab.set_tile(args.lane_bytes, args.block)
+    rng = np.random.default_rng(99)
+    pts = rng.random((P, 3))
+    pos = ea.DeviceBuffer.from_array(pts)
+    packed = args.packed
+    outs = tab.alloc_outputs("vgh", P, packed)
+    wall, kern = time_batches(tab, "vgh", pos, P, outs, args.steps, args.warmup, packed)
+    evals = P * N
+    bpe = bytes_per_eval(4, "vgh", packed, 4)
+    achieved = evals * bpe / (kern * 1e-3) / 1e9
+    log(f"[bench] FP32 vgh: {wall:.4f} ms/step wall, {kern:.4f} ms/launch (events)")
+    traffic = load_traffic("eval3d_f32_vgh_48c_n256_p32768.traffic.json")
+    res = {
+        "metric": METRIC,
+        "value": evals / (wall * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 8 random plane waves per orbital (|k_i|<=4, N(0,1) amplitudes), "
+                "uniform random points, seeded",
+        "config": {"workload": "multi_UBspline_3d_s periodic 48^3, 256 orbitals, vgh at 32768 "
+                               "points (256 walkers x 128 electrons), one launch",
+                   "grid": [M, M, M], "orbitals": N, "points": P, "mode": "vgh",
+                   "table": "f32", "hessian_layout": "packed6" if packed else "full9",
+                   "table_bytes": int(tab.bytes)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "bytes_per_eval": bpe, "kernel_ms": kern,
+                     "note": "table (135.8 MB) fits the 256 MB Infinity Cache; see traffic"},
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle, host cores)")
+        res["cpu_baseline"] = cpu_baseline("s", host, pts, "vgh")
+    # layout variant for the record
+    if not packed:
+        outs_p = tab.alloc_outputs("vgh", P, True)
+        w2, k2 = time_batches(tab, "vgh", pos, P, outs_p, max(5, args.steps // 2), 2, True)
+        res.setdefault("extra", {})["f32_vgh_packed6"] = {
+            "evals_per_s": evals / (w2 * 1e-3), "kernel_ms": k2,
+            "achieved_GBs": evals * bytes_per_eval(4, "vgh", True, 4) / (k2 * 1e-3) / 1e9}
+    return res
+
+
+def precision(args, res):
+    """96^3 periodic, 512 orbitals built in FP64, stored FP32/BF16/FP16;
+    vgh at 131072 points; errors against the FP64 evaluation."""
+    import oracle
+
+    M, N, P = 96, 512, 131072
+    t0 = time.perf_counter()
+    host = build_table("d", M, N, seed=4096, block=16)
+    log(f"[bench] built 96^3 x 512 FP64 table in {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(5)
+    pts = rng.random((P, 3))
+    pos = ea.DeviceBuffer.from_array(pts)
+    sample = pts[:256]
+    o = oracle.Table("d", (M, M, M), (0, 0, 0), (1, 1, 1), N)
+    o.coefs = host.coefs()            # share the FP64 table (read-only)
+    rv, rg, rH = o.eval_batch("vgh", sample, nthreads=min(os.cpu_count() or 1, 16))
+    rows = {}
+    for name, st, cb in (("f32", ea.STORE_F32, 4), ("bf16", ea.STORE_BF16, 2), ("f16", ea.STORE_F16, 2)):
+        tab = ea.DeviceTable(host, storage=st)
+        if name != "f32":
+            tab.set_tile(args.lane_bytes_16 or 0, 0)
+        outs = tab.alloc_outputs("vgh", P, args.packed)
+        wall, kern = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 3, args.packed)
+        # errors on the sample
+        sp = ea.DeviceBuffer.from_array(sample)
+        so = tab.alloc_outputs("vgh", len(sample))
+        tab.eval_batch("vgh", sp, len(sample), so)
+        ea.synchronize()
+        v, g, H = tab.download("vgh", len(sample), so)
+        errs = {}
+        for q, got, ref in (("v", v, rv), ("g", g, rg), ("h", H, rH)):
+            d = got.reshape(ref.shape).astype(np.float64) - ref
+            scale = np.max(np.abs(ref))
+            errs[q] = {"max_rel": float(np.max(np.abs(d)) / scale),
+                       "rms_rel": float(np.sqrt(np.mean(d**2)) / np.sqrt(np.mean(ref**2)))}
+        bpe = bytes_per_eval(cb, "vgh", args.packed, 4)
+        rows[name] = {"evals_per_s": P * N / (wall * 1e-3), "kernel_ms": kern,
+                      "achieved_GBs": P * N * bpe / (kern * 1e-3) / 1e9,
+                      "frac_hbm": P * N * bpe / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "bytes_per_eval": bpe, "table_bytes": int(tab.bytes), "error_vs_f64": errs}
+        log(f"[bench] precision {name}: {kern:.3f} ms/launch, v max rel {errs['v']['max_rel']:.2e}")
+        tab.destroy()
+    rows["bf16_speedup_vs_f32"] = rows["f32"]["kernel_ms"] / rows["bf16"]["kernel_ms"]
+    rows["f16_speedup_vs_f32"] = rows["f32"]["kernel_ms"] / rows["f16"]["kernel_ms"]
+    rows["workload"] = "periodic 96^3, 512 orbitals, vgh at 131072 points; FP64 build"
+    res.setdefault("extra", {})["precision"] = rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", action="store_true", help="add the FP32/BF16/FP16 table comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
+    ap.add_argument("--lane-bytes", type=int, default=0)
+    ap.add_argument("--lane-bytes-16", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the spline evaluator is a single-GPU kernel "
+              "(orbital sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+    if ea.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    res = headline(args)
+    if args.precision:
+        precision(args, res)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,457 @@
+"""einspline_amd -- ctypes front end of libeinspline.so.
+
+The library itself is C (create/set, src/multi_bspline_create.c) and HIP for
+gfx950 (evaluation); this module only declares its C ABI for Python callers
+(tests, benchmarks, serving drivers) and wraps device buffers and streams.
+It does not compute anything itself: if the shared library is missing,
+importing this module raises.
+
+The C declarations mirror include/*.h; see those headers for semantics.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libeinspline.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `make -C einspline_amd/csrc` "
+        "(or __graft_entry__.build())"
+    )
+lib = C.CDLL(LIB_PATH)
+
+# ---------------------------------------------------------------------------
+# types (include/bspline_base.h, include/multi_bspline_structs.h)
+
+PERIODIC, DERIV1, DERIV2, FLAT, NATURAL, ANTIPERIODIC = range(6)
+SINGLE_REAL, DOUBLE_REAL, SINGLE_COMPLEX, DOUBLE_COMPLEX = range(4)
+MULTI_U3D = 8
+STORE_F32, STORE_F64, STORE_BF16, STORE_F16 = range(4)
+MODES = ("v", "vg", "vgl", "vgh")
+
+E_OK, E_INVAL, E_NOMEM, E_HIP, E_RANGE, E_BUDGET, E_BUSY = 0, -1, -2, -3, -4, -5, -6
+
+
+class Ugrid(C.Structure):
+    _fields_ = [("start", C.c_double), ("end", C.c_double), ("num", C.c_int),
+                ("delta", C.c_double), ("delta_inv", C.c_double)]
+
+
+class BCtype_s(C.Structure):
+    _fields_ = [("lCode", C.c_int), ("rCode", C.c_int), ("lVal", C.c_float), ("rVal", C.c_float)]
+
+
+class BCtype_d(C.Structure):
+    _fields_ = [("lCode", C.c_int), ("rCode", C.c_int), ("lVal", C.c_double), ("rVal", C.c_double)]
+
+
+class BCtype_c(C.Structure):
+    _fields_ = [("lCode", C.c_int), ("rCode", C.c_int), ("lVal_r", C.c_float), ("lVal_i", C.c_float),
+                ("rVal_r", C.c_float), ("rVal_i", C.c_float)]
+
+
+class BCtype_z(C.Structure):
+    _fields_ = [("lCode", C.c_int), ("rCode", C.c_int), ("lVal_r", C.c_double), ("lVal_i", C.c_double),
+                ("rVal_r", C.c_double), ("rVal_i", C.c_double)]
+
+
+def _multi(bc, lapl3):
+    f = [("spcode", C.c_int), ("tcode", C.c_int), ("coefs", C.c_void_p),
+         ("x_stride", C.c_ssize_t), ("y_stride", C.c_ssize_t), ("z_stride", C.c_ssize_t),
+         ("x_grid", Ugrid), ("y_grid", Ugrid), ("z_grid", Ugrid),
+         ("xBC", bc), ("yBC", bc), ("zBC", bc), ("num_splines", C.c_int)]
+    if lapl3:
+        f.append(("lapl3", C.c_void_p))
+    return f
+
+
+class multi_UBspline_3d_s(C.Structure):
+    _fields_ = _multi(BCtype_s, False)
+
+
+class multi_UBspline_3d_d(C.Structure):
+    _fields_ = _multi(BCtype_d, False)
+
+
+class multi_UBspline_3d_c(C.Structure):
+    _fields_ = _multi(BCtype_c, True)
+
+
+class multi_UBspline_3d_z(C.Structure):
+    _fields_ = _multi(BCtype_z, True)
+
+
+class multi_UBspline_3d_hip(C.Structure):
+    _fields_ = [("coefs", C.c_void_p),
+                ("x_stride", C.c_ssize_t), ("y_stride", C.c_ssize_t), ("z_stride", C.c_ssize_t),
+                ("x_grid", Ugrid), ("y_grid", Ugrid), ("z_grid", Ugrid),
+                ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int),
+                ("num_splines", C.c_int), ("row_reals", C.c_int),
+                ("tcode", C.c_int), ("storage", C.c_int),
+                ("bytes", C.c_size_t), ("device", C.c_int)]
+
+
+KINDS = {
+    # kind: (struct, BC type, real ctype, numpy real dtype, ncomp)
+    "s": (multi_UBspline_3d_s, BCtype_s, C.c_float, np.float32, 1),
+    "d": (multi_UBspline_3d_d, BCtype_d, C.c_double, np.float64, 1),
+    "c": (multi_UBspline_3d_c, BCtype_c, C.c_float, np.float32, 2),
+    "z": (multi_UBspline_3d_z, BCtype_z, C.c_double, np.float64, 2),
+}
+
+# ---------------------------------------------------------------------------
+# prototypes
+
+_vp, _i, _d, _sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+
+
+def _proto(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+for k, (S, BC, RT, _, _) in KINDS.items():
+    _proto(f"create_multi_UBspline_3d_{k}", C.POINTER(S), Ugrid, Ugrid, Ugrid, BC, BC, BC, _i)
+    _proto(f"set_multi_UBspline_3d_{k}", None, C.POINTER(S), _i, _vp)
+    _proto(f"set_multi_UBspline_3d_{k}_block", None, C.POINTER(S), _i, _i, _vp)
+    _proto(f"eval_multi_UBspline_3d_{k}", None, C.POINTER(S), _d, _d, _d, _vp)
+    _proto(f"eval_multi_UBspline_3d_{k}_vg", None, C.POINTER(S), _d, _d, _d, _vp, _vp)
+    _proto(f"eval_multi_UBspline_3d_{k}_vgl", None, C.POINTER(S), _d, _d, _d, _vp, _vp, _vp)
+    _proto(f"eval_multi_UBspline_3d_{k}_vgh", None, C.POINTER(S), _d, _d, _d, _vp, _vp, _vp)
+    _proto(f"create_multi_UBspline_3d_{k}_hip", C.POINTER(multi_UBspline_3d_hip), C.POINTER(S))
+    _proto(f"eval_multi_multi_UBspline_3d_{k}_hip", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _i)
+    _proto(f"eval_multi_multi_UBspline_3d_{k}_vgh_hip", _i, C.POINTER(multi_UBspline_3d_hip),
+           _vp, _vp, _vp, _vp, _i)
+
+_proto("create_multi_UBspline_3d_s_hip_conv", C.POINTER(multi_UBspline_3d_hip), C.POINTER(multi_UBspline_3d_d))
+_proto("create_multi_UBspline_3d_c_hip_conv", C.POINTER(multi_UBspline_3d_hip), C.POINTER(multi_UBspline_3d_z))
+_proto("create_multi_UBspline_3d_s_hip_store", C.POINTER(multi_UBspline_3d_hip),
+       C.POINTER(multi_UBspline_3d_d), _i)
+_proto("destroy_multi_UBspline_3d_hip", None, C.POINTER(multi_UBspline_3d_hip))
+_proto("destroy_Bspline", None, _vp)
+_proto("einspline_hip_invalidate", None, _vp)
+_proto("einspline_set_tile", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i)
+_proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp)
+_proto("einspline_eval_vg_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp)
+_proto("einspline_eval_vgl_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp)
+_proto("einspline_eval_vgh_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _i, _vp)
+_proto("einspline_hip_device_count", _i)
+_proto("einspline_hip_last_error", C.c_char_p)
+_proto("einspline_hip_malloc", _vp, _sz)
+_proto("einspline_hip_free", _i, _vp)
+_proto("einspline_hip_memcpy_htod", _i, _vp, _vp, _sz)
+_proto("einspline_hip_memcpy_dtoh", _i, _vp, _vp, _sz)
+_proto("einspline_hip_memset", _i, _vp, _i, _sz)
+_proto("einspline_hip_synchronize", _i)
+_proto("einspline_hip_stream_create", _vp)
+_proto("einspline_hip_stream_destroy", _i, _vp)
+_proto("einspline_hip_stream_synchronize", _i, _vp)
+_proto("einspline_hip_event_create", _vp)
+_proto("einspline_hip_event_destroy", _i, _vp)
+_proto("einspline_hip_event_record", _i, _vp, _vp)
+_proto("einspline_hip_event_synchronize", _i, _vp)
+_proto("einspline_hip_event_elapsed_ms", C.c_float, _vp, _vp)
+
+
+class EinsplineError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed ({code}): {lib.einspline_hip_last_error().decode()}")
+        self.code = code
+
+
+def check(rc, what="einspline call"):
+    if rc != 0:
+        raise EinsplineError(rc, what)
+    return rc
+
+
+def device_count() -> int:
+    return lib.einspline_hip_device_count()
+
+
+# ---------------------------------------------------------------------------
+# device memory, streams, events
+
+
+class DeviceBuffer:
+    """Raw device allocation owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib.einspline_hip_malloc(self.nbytes)
+        if not self.ptr:
+            raise EinsplineError(E_NOMEM, f"einspline_hip_malloc({self.nbytes})")
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib.einspline_hip_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "memcpy H2D")
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib.einspline_hip_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "memcpy D2H")
+        return out
+
+    def zero(self):
+        check(lib.einspline_hip_memset(self.ptr, 0, self.nbytes), "memset")
+
+    def free(self):
+        if self.ptr:
+            lib.einspline_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.ptr = lib.einspline_hip_stream_create()
+        if not self.ptr:
+            raise EinsplineError(E_HIP, "stream create")
+
+    def synchronize(self):
+        check(lib.einspline_hip_stream_synchronize(self.ptr), "stream synchronize")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.einspline_hip_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.ptr = lib.einspline_hip_event_create()
+        if not self.ptr:
+            raise EinsplineError(E_HIP, "event create")
+
+    def record(self, stream=None):
+        check(lib.einspline_hip_event_record(self.ptr, stream.ptr if stream else None), "event record")
+
+    def synchronize(self):
+        check(lib.einspline_hip_event_synchronize(self.ptr), "event synchronize")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = lib.einspline_hip_event_elapsed_ms(self.ptr, end.ptr)
+        if ms < 0:
+            raise EinsplineError(E_HIP, "event elapsed")
+        return ms
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.einspline_hip_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib.einspline_hip_synchronize(), "device synchronize")
+
+
+# ---------------------------------------------------------------------------
+# host splines (create/set run on the host; single-point evals on the GPU)
+
+
+def _bc(kind, lcode, rcode, lval=0.0, rval=0.0):
+    BC = KINDS[kind][1]
+    if kind in "sd":
+        return BC(lcode, rcode, float(np.real(lval)), float(np.real(rval)))
+    return BC(lcode, rcode, float(np.real(lval)), float(np.imag(lval)),
+              float(np.real(rval)), float(np.imag(rval)))
+
+
+class HostSpline:
+    """A multi_UBspline_3d_{s,d,c,z} created by libeinspline.
+
+    M, start, end: per-axis grid (3-sequences); bc: per-axis
+    (lcode, rcode[, lval, rval]) tuples, default periodic.
+    """
+
+    def __init__(self, kind, M, start, end, num_splines, bc=None):
+        S, BC, RT, dt, nc = KINDS[kind]
+        self.kind, self.dtype, self.ncomp = kind, dt, nc
+        self.N = int(num_splines)
+        bc = bc or [(PERIODIC, PERIODIC)] * 3
+        grids = [Ugrid(float(s), float(e), int(m), 0.0, 0.0) for m, s, e in zip(M, start, end)]
+        bcs = [_bc(kind, *b) for b in bc]
+        self.ptr = getattr(lib, f"create_multi_UBspline_3d_{kind}")(*grids, *bcs, self.N)
+        if not self.ptr:
+            raise EinsplineError(E_INVAL, f"create_multi_UBspline_3d_{kind}")
+        st = self.ptr.contents
+        self.M = tuple(int(m) for m in M)
+        self.shape = tuple(int(m) + (3 if b[0] in (PERIODIC, ANTIPERIODIC) else 2) for m, b in zip(M, bc))
+
+    @property
+    def struct(self):
+        return self.ptr.contents
+
+    def coefs(self) -> np.ndarray:
+        """Zero-copy view of the host table, shape (Nx, Ny, Nz, N[, 2])."""
+        n = int(np.prod(self.shape)) * self.N * self.ncomp
+        RT = KINDS[self.kind][2]
+        buf = (RT * n).from_address(self.struct.coefs)
+        a = np.ctypeslib.as_array(buf)
+        shp = self.shape + (self.N,) + ((2,) if self.ncomp == 2 else ())
+        return a.reshape(shp)
+
+    def set(self, n, data):
+        data = self._data(data, 1)
+        getattr(lib, f"set_multi_UBspline_3d_{self.kind}")(self.ptr, int(n), data.ctypes.data)
+
+    def set_block(self, first, data):
+        """data: (count, Mx, My, Mz) real or complex array."""
+        count = data.shape[0]
+        data = self._data(data, count)
+        getattr(lib, f"set_multi_UBspline_3d_{self.kind}_block")(self.ptr, int(first), int(count),
+                                                                 data.ctypes.data)
+
+    def _data(self, data, count):
+        data = np.asarray(data)
+        if self.ncomp == 2:
+            cdt = np.complex64 if self.kind == "c" else np.complex128
+            data = np.ascontiguousarray(data, dtype=cdt)
+        else:
+            data = np.ascontiguousarray(data, dtype=self.dtype)
+        assert data.size == count * int(np.prod(self.M)), "data must hold Mx*My*Mz values per orbital"
+        return data
+
+    def invalidate(self):
+        lib.einspline_hip_invalidate(self.ptr)
+
+    def eval(self, mode, x, y, z):
+        """Single-point evaluation through eval_multi_UBspline_3d_<kind>[_mode]."""
+        N, nc = self.N, self.ncomp
+        cdt = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}[self.kind]
+        v = np.zeros(N, cdt)
+        g = np.zeros(3 * N, cdt)
+        h = np.zeros(9 * N if mode == "vgh" else N, cdt)
+        suf = "" if mode == "v" else "_" + mode
+        f = getattr(lib, f"eval_multi_UBspline_3d_{self.kind}{suf}")
+        args = [v.ctypes.data] + ([g.ctypes.data] if mode != "v" else []) + \
+               ([h.ctypes.data] if mode in ("vgl", "vgh") else [])
+        f(self.ptr, float(x), float(y), float(z), *args)
+        out = [v]
+        if mode != "v":
+            out.append(g.reshape(N, 3))
+        if mode == "vgl":
+            out.append(h)
+        if mode == "vgh":
+            out.append(h.reshape(N, 3, 3))
+        return tuple(out)
+
+    def destroy(self):
+        if self.ptr:
+            lib.destroy_Bspline(C.cast(self.ptr, C.c_void_p))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# device tables and batched evaluation
+
+
+class DeviceTable:
+    """A multi_UBspline_3d_hip built from a HostSpline.
+
+    storage: None (the host type), or STORE_F32/BF16/F16 from a 'd' spline
+    (precision form), or 'conv' to convert d->s / z->c.
+    """
+
+    def __init__(self, host: HostSpline, storage=None):
+        k = host.kind
+        if storage is None:
+            p = getattr(lib, f"create_multi_UBspline_3d_{k}_hip")(host.ptr)
+        elif storage == "conv":
+            p = getattr(lib, {"d": "create_multi_UBspline_3d_s_hip_conv",
+                              "z": "create_multi_UBspline_3d_c_hip_conv"}[k])(host.ptr)
+        else:
+            assert k == "d", "storage conversion starts from a double table"
+            p = lib.create_multi_UBspline_3d_s_hip_store(host.ptr, int(storage))
+        if not p:
+            raise EinsplineError(E_INVAL, "create device table")
+        self.ptr = p
+        t = p.contents
+        self.N = t.num_splines
+        self.tcode = t.tcode
+        self.storage = t.storage
+        self.complex = t.tcode in (SINGLE_COMPLEX, DOUBLE_COMPLEX)
+        self.result_dtype = np.float64 if t.storage == STORE_F64 else np.float32
+        self.nreal = self.N * (2 if self.complex else 1)
+        self.bytes = t.bytes
+
+    def set_tile(self, lane_bytes=0, block=0):
+        check(lib.einspline_set_tile(self.ptr, int(lane_bytes), int(block)), "einspline_set_tile")
+
+    def out_sizes(self, mode, packed=False):
+        """Reals per point of (vals, grads, lapl/hess)."""
+        n = self.nreal
+        h = {"v": 0, "vg": 0, "vgl": n, "vgh": (6 if packed else 9) * n}[mode]
+        return n, (3 * n if mode != "v" else 0), h
+
+    def alloc_outputs(self, mode, npts, packed=False):
+        es = np.dtype(self.result_dtype).itemsize
+        return tuple(DeviceBuffer(max(1, s * npts * es)) if s else None
+                     for s in self.out_sizes(mode, packed))
+
+    def eval_batch(self, mode, pos: DeviceBuffer, npts, outs, packed=False, stream=None):
+        v, g, h = outs
+        sp = stream.ptr if stream else None
+        if mode == "v":
+            rc = lib.einspline_eval_v_batch(self.ptr, pos.ptr, npts, v.ptr, sp)
+        elif mode == "vg":
+            rc = lib.einspline_eval_vg_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, sp)
+        elif mode == "vgl":
+            rc = lib.einspline_eval_vgl_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, h.ptr, sp)
+        else:
+            rc = lib.einspline_eval_vgh_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, h.ptr,
+                                              1 if packed else 0, sp)
+        check(rc, f"einspline_eval_{mode}_batch")
+
+    def download(self, mode, npts, outs, packed=False):
+        """Copy batch results to host arrays shaped per point (complex for c/z)."""
+        res = []
+        for buf, s in zip(outs, self.out_sizes(mode, packed)):
+            if not s:
+                continue
+            a = buf.download(self.result_dtype, (npts, s))
+            if self.complex:
+                a = a.view(np.complex64 if self.result_dtype == np.float32 else np.complex128)
+            res.append(a)
+        return tuple(res)
+
+    def destroy(self):
+        if self.ptr:
+            lib.destroy_multi_UBspline_3d_hip(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

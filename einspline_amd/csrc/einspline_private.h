@@ -1,0 +1,46 @@
+/* einspline_private.h -- library-internal layout shared by the C host code
+ * (create/set/destroy) and the HIP runtime. */
+#ifndef EINSPLINE_PRIVATE_H
+#define EINSPLINE_PRIVATE_H
+
+#include <pthread.h>
+#include <stdint.h>
+#include "multi_bspline_structs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EINSPLINE_HOST_MAGIC 0x45534833u /* "ESH3" */
+
+/* Every 3-D multi-spline returned by create_multi_UBspline_3d_* is the
+ * public struct followed by this private tail, so a pointer to the public
+ * struct is a pointer to the whole object. */
+typedef struct einspline_host3d {
+  union {
+    multi_UBspline_3d_s s;
+    multi_UBspline_3d_d d;
+    multi_UBspline_3d_c c;
+    multi_UBspline_3d_z z;
+  } pub;
+  uint32_t magic;
+  int Nx, Ny, Nz;          /* coefficient-table extent per axis */
+  int ncomp;               /* reals per element: 1 real, 2 complex */
+  int nsplines;            /* num_splines (its offset differs between the union members) */
+  int dirty;               /* host coefs changed since the device mirror was built */
+  void *mirror;            /* multi_UBspline_3d_hip*, built on first single-point eval */
+  pthread_mutex_t lock;    /* guards dirty/mirror */
+} einspline_host3d;
+
+/* Returns the private object behind a public pointer, or NULL if the
+ * pointer was not created by this library. */
+einspline_host3d *einspline_host_of(void *spline);
+
+/* Implemented by the HIP runtime: releases a device mirror. */
+void einspline_mirror_release(void *mirror);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

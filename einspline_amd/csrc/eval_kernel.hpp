@@ -1,0 +1,507 @@
+// eval_kernel.hpp -- gfx950 kernels for batched 3-D multi-spline evaluation.
+//
+// One work-item ("lane") owns one (point, chunk) pair, where a chunk is
+// LB contiguous bytes (V = LB / sizeof(storage) reals) of an orbital row.
+// Lanes of the same point are consecutive, so for each of the 64 stencil
+// rows a wavefront issues one fully coalesced load of 64*LB bytes.  The
+// contraction follows src/multi_bspline_eval_std_d_impl.h:742-864 (the
+// reference's std vgh loop) but is sum-factorised over the z axis: for every
+// (i,j) the four contiguous k-rows are first reduced with the z basis
+// (c, dc, d2c), then combined with the (i,j) products of the x/y basis.  That
+// is 22 FMAs per coefficient quad for vgh instead of 40.
+//
+// Position -> cell follows the std evaluators: u = (x - start) * delta_inv
+// in double, rounded to the accumulation type for float tables
+// (src/multi_bspline_eval_std_s_impl.h:749-755), integer part by truncation,
+// t = u - trunc(u).  The cell index is clamped to [0, N-4] with t adjusted,
+// which is the identity for points inside [start, end] and keeps every
+// access in bounds otherwise.
+//
+// Complex tables are handled as real tables of 2N interleaved reals (the
+// basis is real); only the output permutation differs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace esk {
+
+#ifndef ESK_WAVES_PER_EU
+#define ESK_WAVES_PER_EU 4
+#endif
+#ifndef ESK_PREFETCH
+#define ESK_PREFETCH 1
+#endif
+
+enum { MODE_V = 0, MODE_VG = 1, MODE_VGL = 2, MODE_VGH = 3 };
+
+// Storage tags for 16-bit tables (accumulated in float).
+struct bf16s { uint16_t b; };
+struct f16s  { uint16_t b; };
+
+template <class S> struct acc_of        { using type = S; };
+template <>        struct acc_of<bf16s> { using type = float; };
+template <>        struct acc_of<f16s>  { using type = float; };
+
+// Kernel arguments (passed by value; ~200 bytes).
+struct Params {
+  const void *coefs;         // device table base
+  long xs, ys, zs;           // strides in storage elements (reals)
+  double x0, y0, z0;         // grid starts
+  double xi, yi, zi;         // delta_inv per axis
+  int nx4, ny4, nz4;         // N-4 per axis: largest valid cell index
+  int chunks;                // lanes per point (row_reals / V)
+  int nreal;                 // valid reals per point (N, or 2N complex)
+  int npoints;
+  const void *pos;           // positions, 3 per point (double or float)
+  void *out_v, *out_g, *out_h;   // strided bases, or device arrays of per-point pointers
+  long ld_v, ld_g, ld_h;     // per-point strides of strided outputs, in reals
+  int hpacked;               // vgh: 1 -> 6 unique Hessian entries per orbital, 0 -> full 9
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+
+template <class T> __device__ __forceinline__ T uniform(T v);
+template <> __device__ __forceinline__ int uniform<int>(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+template <> __device__ __forceinline__ float uniform<float>(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+template <> __device__ __forceinline__ double uniform<double>(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float fmaA(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaA(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float truncA(float a) { return __builtin_truncf(a); }
+__device__ __forceinline__ double truncA(double a) { return __builtin_trunc(a); }
+
+// Empty asm that claims to modify v: forces v to be computed at this point.
+__device__ __forceinline__ void pin(float &v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
+
+// Cell index and fraction along one axis.
+template <class A>
+__device__ __forceinline__ void locate(double x, double x0, double xinv, int n4,
+                                       int &ix, A &t) {
+  A u = (A)((x - x0) * xinv);
+  A ip = truncA(u);
+  A ipc = ip < (A)0 ? (A)0 : ip;          // NaN compares false -> kept, fixed below
+  ipc = ipc > (A)n4 ? (A)n4 : ipc;
+  if (!(ipc == ipc)) ipc = (A)0;
+  t = u - ipc;
+  ix = (int)ipc;
+}
+
+// Uniform cubic B-spline basis (power form, rows of the 4x4 basis matrix in
+// src/bspline_data.c:143-193) and its first and second t-derivatives.
+template <class A>
+__device__ __forceinline__ void basis0(A t, A w[4]) {
+  const A s6 = (A)(1.0 / 6.0), h = (A)0.5, t3 = (A)(2.0 / 3.0);
+  w[0] = ((-s6 * t + h) * t - h) * t + s6;
+  w[1] = (h * t - (A)1) * t * t + t3;
+  w[2] = ((-h * t + h) * t + h) * t + s6;
+  w[3] = s6 * t * t * t;
+}
+template <class A>
+__device__ __forceinline__ void basis1(A t, A w[4]) {
+  const A h = (A)0.5, o5 = (A)1.5;
+  w[0] = (-h * t + (A)1) * t - h;
+  w[1] = (o5 * t - (A)2) * t;
+  w[2] = (-o5 * t + (A)1) * t + h;
+  w[3] = h * t * t;
+}
+template <class A>
+__device__ __forceinline__ void basis2(A t, A w[4]) {
+  w[0] = (A)1 - t;
+  w[1] = (A)3 * t - (A)2;
+  w[2] = (A)1 - (A)3 * t;
+  w[3] = t;
+}
+
+// Load V reals of storage type S (LB bytes) and widen to A.
+template <class S, int LB> struct Loader;
+
+template <int LB> struct Loader<float, LB> {
+  static constexpr int V = LB / 4;
+  __device__ __forceinline__ static void load(const float *p, float *r) {
+    if constexpr (LB == 4) { r[0] = p[0]; }
+    else if constexpr (LB == 8) { float2 q = *(const float2 *)p; r[0] = q.x; r[1] = q.y; }
+    else {
+#pragma unroll
+      for (int c = 0; c < LB / 16; ++c) {
+        float4 q = *((const float4 *)p + c);
+        r[4 * c + 0] = q.x; r[4 * c + 1] = q.y; r[4 * c + 2] = q.z; r[4 * c + 3] = q.w;
+      }
+    }
+  }
+};
+template <int LB> struct Loader<double, LB> {
+  static constexpr int V = LB / 8;
+  __device__ __forceinline__ static void load(const double *p, double *r) {
+    if constexpr (LB == 8) { r[0] = p[0]; }
+    else {
+#pragma unroll
+      for (int c = 0; c < LB / 16; ++c) {
+        double2 q = *((const double2 *)p + c);
+        r[2 * c + 0] = q.x; r[2 * c + 1] = q.y;
+      }
+    }
+  }
+};
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ float f16_lo(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+}
+__device__ __forceinline__ float f16_hi(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+template <int LB> struct Loader<bf16s, LB> {
+  static constexpr int V = LB / 2;
+  __device__ __forceinline__ static void load(const bf16s *p, float *r) {
+    if constexpr (LB == 4) { uint32_t w = *(const uint32_t *)p; r[0] = bf16_lo(w); r[1] = bf16_hi(w); }
+    else if constexpr (LB == 8) {
+      uint2 q = *(const uint2 *)p;
+      r[0] = bf16_lo(q.x); r[1] = bf16_hi(q.x); r[2] = bf16_lo(q.y); r[3] = bf16_hi(q.y);
+    } else {
+#pragma unroll
+      for (int c = 0; c < LB / 16; ++c) {
+        uint4 q = *((const uint4 *)p + c);
+        r[8 * c + 0] = bf16_lo(q.x); r[8 * c + 1] = bf16_hi(q.x);
+        r[8 * c + 2] = bf16_lo(q.y); r[8 * c + 3] = bf16_hi(q.y);
+        r[8 * c + 4] = bf16_lo(q.z); r[8 * c + 5] = bf16_hi(q.z);
+        r[8 * c + 6] = bf16_lo(q.w); r[8 * c + 7] = bf16_hi(q.w);
+      }
+    }
+  }
+};
+template <int LB> struct Loader<f16s, LB> {
+  static constexpr int V = LB / 2;
+  __device__ __forceinline__ static void load(const f16s *p, float *r) {
+    if constexpr (LB == 4) { uint32_t w = *(const uint32_t *)p; r[0] = f16_lo(w); r[1] = f16_hi(w); }
+    else if constexpr (LB == 8) {
+      uint2 q = *(const uint2 *)p;
+      r[0] = f16_lo(q.x); r[1] = f16_hi(q.x); r[2] = f16_lo(q.y); r[3] = f16_hi(q.y);
+    } else {
+#pragma unroll
+      for (int c = 0; c < LB / 16; ++c) {
+        uint4 q = *((const uint4 *)p + c);
+        r[8 * c + 0] = f16_lo(q.x); r[8 * c + 1] = f16_hi(q.x);
+        r[8 * c + 2] = f16_lo(q.y); r[8 * c + 3] = f16_hi(q.y);
+        r[8 * c + 4] = f16_lo(q.z); r[8 * c + 5] = f16_hi(q.z);
+        r[8 * c + 6] = f16_lo(q.w); r[8 * c + 7] = f16_hi(q.w);
+      }
+    }
+  }
+};
+
+// Streaming stores: results are written once and never re-read by the
+// kernel, so they bypass the caches that hold the coefficient table.
+template <class T>
+__device__ __forceinline__ void st1(T *p, T v) { __builtin_nontemporal_store(v, p); }
+
+// Store n = K reals held in r[] to dst (contiguous).  `vec` selects 16-byte
+// stores (dst must then be 16-byte aligned and K*sizeof(A) a multiple of 16).
+template <class A, int K>
+__device__ __forceinline__ void store_block(A *dst, const A *r, bool vec) {
+  constexpr int PER16 = 16 / sizeof(A);
+  if constexpr ((K * sizeof(A)) % 16 == 0) {
+    if (vec) {
+      typedef A v16 __attribute__((ext_vector_type(PER16)));
+#pragma unroll
+      for (int c = 0; c < K / PER16; ++c) {
+        v16 q;
+#pragma unroll
+        for (int e = 0; e < PER16; ++e) q[e] = r[c * PER16 + e];
+        __builtin_nontemporal_store(q, (v16 *)dst + c);
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < K; ++e) st1(dst + e, r[e]);
+}
+
+// Store only the first m (< K) reals.
+template <class A, int K>
+__device__ __forceinline__ void store_partial(A *dst, const A *r, int m) {
+#pragma unroll
+  for (int e = 0; e < K; ++e)
+    if (e < m) st1(dst + e, r[e]);
+}
+
+// ---------------------------------------------------------------------------
+// the kernel
+//
+// S       storage type (float, double, bf16s, f16s)
+// LB      bytes of one row that a lane loads per stencil row (4..32)
+// MODE    MODE_V / MODE_VG / MODE_VGL / MODE_VGH
+// CPLX    complex orbitals (interleaved re/im reals)
+// UNIF    every wavefront serves exactly one point (chunks % 64 == 0)
+// PosT    position element type (double or float)
+// PTRS    outputs are device arrays of per-point pointers (CUDA-style API)
+
+// Occupancy target from an estimate of the live registers of one
+// instantiation: accumulators, in-flight rows, z-reduced partials, basis
+// weights (only when they cannot live in scalar registers) and addressing.
+template <class S, int LB, int MODE, bool UNIF>
+struct Occupancy {
+  using A = typename acc_of<S>::type;
+  static constexpr int V = LB / (int)sizeof(S);
+  static constexpr int W = (int)sizeof(A) / 4;              // VGPRs per value
+  static constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7 : 10;
+  static constexpr int NW = MODE == MODE_V ? 12 : MODE == MODE_VG ? 24 : 36;
+  // double weights overflow the scalar register file, so half of them are
+  // counted as vector registers even for one point per wavefront
+  static constexpr int regs = NACC * V * W + (ESK_PREFETCH + 1) * 4 * V * W + 3 * V * W +
+                              (UNIF ? (W == 2 ? NW : 0) : NW * W) + 24;
+  static constexpr int waves = regs <= 116 ? ESK_WAVES_PER_EU : regs <= 160 ? 3 : regs <= 248 ? 2 : 1;
+};
+
+template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS>
+__global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) eval3d(const Params P) {
+  using A = typename acc_of<S>::type;
+  using L = Loader<S, LB>;
+  constexpr int V = L::V;
+  static_assert(!CPLX || (V % 2 == 0), "complex lanes hold whole orbitals");
+
+  // npoints * chunks < 2^31 is checked on the host, so 32-bit math suffices
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = (int)(gid / (unsigned)P.chunks);
+  if (UNIF) p = uniform(p);
+  const int ch = (int)(gid - (unsigned)p * (unsigned)P.chunks);
+  if (p >= P.npoints) return;
+
+  const PosT *pp = (const PosT *)P.pos + 3l * p;
+  const double x = (double)pp[0], y = (double)pp[1], z = (double)pp[2];
+  int ix, iy, iz;
+  A tx, ty, tz;
+  locate<A>(x, P.x0, P.xi, P.nx4, ix, tx);
+  locate<A>(y, P.y0, P.yi, P.ny4, iy, ty);
+  locate<A>(z, P.z0, P.zi, P.nz4, iz, tz);
+  if (UNIF) {
+    ix = uniform(ix); iy = uniform(iy); iz = uniform(iz);
+    tx = uniform(tx); ty = uniform(ty); tz = uniform(tz);
+  }
+
+  A a[4], b[4], c[4], da[4], db[4], dc[4], d2a[4], d2b[4], d2c[4];
+  basis0(tx, a); basis0(ty, b); basis0(tz, c);
+  if constexpr (MODE >= MODE_VG) { basis1(tx, da); basis1(ty, db); basis1(tz, dc); }
+  if constexpr (MODE >= MODE_VGL) { basis2(tx, d2a); basis2(ty, d2b); basis2(tz, d2c); }
+  if (UNIF) {
+    // one point per wavefront: keep the basis in scalar registers
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = uniform(a[q]); b[q] = uniform(b[q]); c[q] = uniform(c[q]);
+      if constexpr (MODE >= MODE_VG) { da[q] = uniform(da[q]); db[q] = uniform(db[q]); dc[q] = uniform(dc[q]); }
+      if constexpr (MODE >= MODE_VGL) { d2a[q] = uniform(d2a[q]); d2b[q] = uniform(d2b[q]); d2c[q] = uniform(d2c[q]); }
+    }
+  }
+
+  // accumulators
+  constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7 : 10;
+  A acc[NACC][V];
+#pragma unroll
+  for (int q = 0; q < NACC; ++q)
+#pragma unroll
+    for (int e = 0; e < V; ++e) acc[q][e] = (A)0;
+
+  const S *base = (const S *)P.coefs + (long)ix * P.xs + (long)iy * P.ys + (long)iz * P.zs +
+                  (long)ch * V;
+  const long zs = P.zs;
+
+  // 16 (i,j) steps, fully unrolled so the basis indices are constants.  The
+  // rows of step st+PF are requested while step st is reduced, and a
+  // scheduling barrier after each step stops the compiler from hoisting all
+  // 64 row loads (which would need ~256 VGPRs and kill occupancy).
+  constexpr int PF = ESK_PREFETCH;
+  A rb[PF + 1][4][V];
+#pragma unroll
+  for (int st = 0; st < PF; ++st) {
+    const S *rp = base + (st >> 2) * P.xs + (st & 3) * P.ys;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L::load(rp + k * zs, rb[st][k]);
+  }
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int i = st >> 2, j = st & 3;
+    if (st + PF < 16) {
+      const int nx = st + PF;
+      const S *rp = base + (nx >> 2) * P.xs + (nx & 3) * P.ys;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) L::load(rp + k * zs, rb[nx % (PF + 1)][k]);
+    }
+    A (&r)[4][V] = rb[st % (PF + 1)];
+
+    A s0[V], s1[V], s2[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      s0[e] = c[0] * r[0][e];
+      s0[e] = fmaA(c[1], r[1][e], s0[e]);
+      s0[e] = fmaA(c[2], r[2][e], s0[e]);
+      s0[e] = fmaA(c[3], r[3][e], s0[e]);
+      if constexpr (MODE >= MODE_VG) {
+        s1[e] = dc[0] * r[0][e];
+        s1[e] = fmaA(dc[1], r[1][e], s1[e]);
+        s1[e] = fmaA(dc[2], r[2][e], s1[e]);
+        s1[e] = fmaA(dc[3], r[3][e], s1[e]);
+      }
+      if constexpr (MODE >= MODE_VGL) {
+        s2[e] = d2c[0] * r[0][e];
+        s2[e] = fmaA(d2c[1], r[1][e], s2[e]);
+        s2[e] = fmaA(d2c[2], r[2][e], s2[e]);
+        s2[e] = fmaA(d2c[3], r[3][e], s2[e]);
+      }
+    }
+    const A ab = a[i] * b[j];
+    if constexpr (MODE == MODE_V) {
+#pragma unroll
+      for (int e = 0; e < V; ++e) acc[0][e] = fmaA(ab, s0[e], acc[0][e]);
+    } else {
+      const A dab = da[i] * b[j], adb = a[i] * db[j];
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        acc[0][e] = fmaA(ab, s0[e], acc[0][e]);    // v
+        acc[1][e] = fmaA(dab, s0[e], acc[1][e]);   // d/dx
+        acc[2][e] = fmaA(adb, s0[e], acc[2][e]);   // d/dy
+        acc[3][e] = fmaA(ab, s1[e], acc[3][e]);    // d/dz
+      }
+      if constexpr (MODE == MODE_VGL) {
+        const A d2ab = d2a[i] * b[j], ad2b = a[i] * d2b[j];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          acc[4][e] = fmaA(d2ab, s0[e], acc[4][e]);  // d2/dx2
+          acc[5][e] = fmaA(ad2b, s0[e], acc[5][e]);  // d2/dy2
+          acc[6][e] = fmaA(ab, s2[e], acc[6][e]);    // d2/dz2
+        }
+      }
+      if constexpr (MODE == MODE_VGH) {
+        const A d2ab = d2a[i] * b[j], dadb = da[i] * db[j], ad2b = a[i] * d2b[j];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          acc[4][e] = fmaA(d2ab, s0[e], acc[4][e]);  // xx
+          acc[5][e] = fmaA(dadb, s0[e], acc[5][e]);  // xy
+          acc[6][e] = fmaA(dab, s1[e], acc[6][e]);   // xz
+          acc[7][e] = fmaA(ad2b, s0[e], acc[7][e]);  // yy
+          acc[8][e] = fmaA(adb, s1[e], acc[8][e]);   // yz
+          acc[9][e] = fmaA(ab, s2[e], acc[9][e]);    // zz
+        }
+      }
+    }
+    // Materialise the accumulators here: without this the optimiser sinks
+    // the reductions past later loads and keeps every loaded row alive.
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+#pragma unroll
+      for (int e = 0; e < V; ++e) pin(acc[q][e]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // scale derivatives by delta_inv (rounded to A first, as the reference does)
+  const A dxi = (A)P.xi, dyi = (A)P.yi, dzi = (A)P.zi;
+  if constexpr (MODE >= MODE_VG) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      acc[1][e] *= dxi; acc[2][e] *= dyi; acc[3][e] *= dzi;
+    }
+  }
+  if constexpr (MODE == MODE_VGL) {
+    const A sxx = dxi * dxi, syy = dyi * dyi, szz = dzi * dzi;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      A lx = acc[4][e] * sxx, ly = acc[5][e] * syy, lz = acc[6][e] * szz;
+      acc[4][e] = lx + ly + lz;
+    }
+  }
+  if constexpr (MODE == MODE_VGH) {
+    const A sxx = dxi * dxi, sxy = dxi * dyi, sxz = dxi * dzi;
+    const A syy = dyi * dyi, syz = dyi * dzi, szz = dzi * dzi;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      acc[4][e] *= sxx; acc[5][e] *= sxy; acc[6][e] *= sxz;
+      acc[7][e] *= syy; acc[8][e] *= syz; acc[9][e] *= szz;
+    }
+  }
+
+  // ---- outputs ----
+  A *ov, *og = nullptr, *oh = nullptr;
+  if constexpr (PTRS) {
+    ov = ((A *const *)P.out_v)[p];
+    if constexpr (MODE >= MODE_VG) og = ((A *const *)P.out_g)[p];
+    if constexpr (MODE >= MODE_VGL) oh = ((A *const *)P.out_h)[p];
+  } else {
+    ov = (A *)P.out_v + (long)p * P.ld_v;
+    if constexpr (MODE >= MODE_VG) og = (A *)P.out_g + (long)p * P.ld_g;
+    if constexpr (MODE >= MODE_VGL) oh = (A *)P.out_h + (long)p * P.ld_h;
+  }
+  uintptr_t align = (uintptr_t)ov;
+  if constexpr (MODE >= MODE_VG) align |= (uintptr_t)og;
+  if constexpr (MODE >= MODE_VGL) align |= (uintptr_t)oh;
+  const bool vec = (align & 15) == 0;
+
+  const int e0 = ch * V;                 // first real of this lane
+  const int m = P.nreal - e0;            // valid reals in this lane (may exceed V)
+  const bool full = m >= V;
+
+  // values (and the Laplacian) keep the real index layout
+  {
+    A r[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) r[e] = acc[0][e];
+    if (full) store_block<A, V>(ov + e0, r, vec); else store_partial<A, V>(ov + e0, r, m);
+  }
+  if constexpr (MODE == MODE_VGL) {
+    A r[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) r[e] = acc[4][e];
+    if (full) store_block<A, V>(oh + e0, r, vec); else store_partial<A, V>(oh + e0, r, m);
+  }
+  if constexpr (MODE >= MODE_VG) {
+    // gradient: real orbital e -> [e][3]; complex orbital q -> [q][3][re,im]
+    A r[3 * V];
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int o = CPLX ? (e >> 1) * 6 + d * 2 + (e & 1) : e * 3 + d;
+        r[o] = acc[1 + d][e];
+      }
+    if (full) store_block<A, 3 * V>(og + 3 * e0, r, vec);
+    else store_partial<A, 3 * V>(og + 3 * e0, r, 3 * m);
+  }
+  if constexpr (MODE == MODE_VGH) {
+    // acc index of each Hessian entry, row-major 3x3 and packed upper triangle
+    constexpr int full9[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
+    constexpr int pack6[6] = {4, 5, 6, 7, 8, 9};
+    if (P.hpacked) {
+      A r[6 * V];
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+#pragma unroll
+        for (int h = 0; h < 6; ++h) {
+          const int o = CPLX ? (e >> 1) * 12 + h * 2 + (e & 1) : e * 6 + h;
+          r[o] = acc[pack6[h]][e];
+        }
+      if (full) store_block<A, 6 * V>(oh + 6 * e0, r, vec);
+      else store_partial<A, 6 * V>(oh + 6 * e0, r, 6 * m);
+    } else {
+      A r[9 * V];
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+          const int o = CPLX ? (e >> 1) * 18 + h * 2 + (e & 1) : e * 9 + h;
+          r[o] = acc[full9[h]][e];
+        }
+      if (full) store_block<A, 9 * V>(oh + 9 * e0, r, vec);
+      else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
+    }
+  }
+}
+
+}  // namespace esk

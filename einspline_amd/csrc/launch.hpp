@@ -1,0 +1,77 @@
+// launch.hpp -- host-side dispatch from runtime parameters to one
+// instantiation of esk::eval3d.  Each storage type is instantiated in its own
+// translation unit (launch_<type>.hip) so the kernels build in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "eval_kernel.hpp"
+
+namespace esk {
+
+enum Api { API_STRIDED = 0, API_PTRS_F32POS = 1, API_PTRS_F64POS = 2 };
+
+struct Launch {
+  int mode;          // MODE_*
+  bool cplx;
+  int api;           // Api
+  int lane_bytes;    // 8, 16 or 32 (16-bit tables also 8..32)
+  int block;         // work-items per block, multiple of 64, <= 256
+};
+
+hipError_t launch_f32(const Launch &L, const Params &P, hipStream_t s);
+hipError_t launch_f64(const Launch &L, const Params &P, hipStream_t s);
+hipError_t launch_bf16(const Launch &L, const Params &P, hipStream_t s);
+hipError_t launch_f16(const Launch &L, const Params &P, hipStream_t s);
+
+#ifdef ESK_INSTANTIATE
+// Instantiation helpers, used only by launch_<type>.hip.
+template <class S, int LB, int MODE, bool CPLX, bool UNIF>
+hipError_t go_api(const Launch &L, const Params &P, hipStream_t s) {
+  const long total = (long)P.npoints * P.chunks;
+  dim3 block(L.block), grid((unsigned)((total + L.block - 1) / L.block));
+  if (L.api == API_STRIDED) {
+    hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, double, false>), grid, block, 0, s, P);
+    return hipGetLastError();
+  }
+  if constexpr (LB == 16 && (MODE == MODE_V || MODE == MODE_VGH)) {
+    if (L.api == API_PTRS_F32POS) {
+      hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, float, true>), grid, block, 0, s, P);
+      return hipGetLastError();
+    }
+    if (L.api == API_PTRS_F64POS) {
+      hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, double, true>), grid, block, 0, s, P);
+      return hipGetLastError();
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class S, int LB, int MODE, bool CPLX>
+hipError_t go_unif(const Launch &L, const Params &P, hipStream_t s) {
+  const bool unif = (P.chunks % 64) == 0 && (L.block % 64) == 0;
+  return unif ? go_api<S, LB, MODE, CPLX, true>(L, P, s)
+              : go_api<S, LB, MODE, CPLX, false>(L, P, s);
+}
+
+template <class S, int LB, bool CPLX>
+hipError_t go_mode(const Launch &L, const Params &P, hipStream_t s) {
+  switch (L.mode) {
+    case MODE_V:   return go_unif<S, LB, MODE_V, CPLX>(L, P, s);
+    case MODE_VG:  return go_unif<S, LB, MODE_VG, CPLX>(L, P, s);
+    case MODE_VGL: return go_unif<S, LB, MODE_VGL, CPLX>(L, P, s);
+    case MODE_VGH: return go_unif<S, LB, MODE_VGH, CPLX>(L, P, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class S, int LB>
+hipError_t go_cplx(const Launch &L, const Params &P, hipStream_t s) {
+  if constexpr (sizeof(S) >= 4) {
+    if (L.cplx) return go_mode<S, LB, true>(L, P, s);
+  } else {
+    if (L.cplx) return hipErrorInvalidValue;
+  }
+  return go_mode<S, LB, false>(L, P, s);
+}
+#endif
+
+}  // namespace esk

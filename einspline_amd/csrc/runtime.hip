@@ -1,0 +1,623 @@
+// runtime.hip -- device tables, batched entry points, the single-point
+// path and a minimal device runtime, all behind the C ABI declared in
+// include/multi_bspline_hip.h and include/multi_bspline_eval_*.h.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "multi_bspline.h"
+#include "einspline_private.h"
+#include "launch.hpp"
+
+using esk::Launch;
+using esk::Params;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local char g_err[512] = "no error";
+
+static int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+  return fail(EINSPLINE_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+extern "C" const char *einspline_hip_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// host-side conversion to the storage types (round to nearest even)
+
+// Round a finite non-negative double to a binary format with `mant` explicit
+// mantissa bits, exponent bias `bias` and largest biased exponent `emax`.
+// Returns the bit pattern without sign; sets *ovf on overflow to infinity.
+static uint32_t round_small(double a, int mant, int bias, int emax, int *ovf) {
+  if (a == 0.0) return 0;
+  int e;
+  frexp(a, &e);                               // a = f * 2^e, f in [0.5, 1)
+  int E = e - 1 + bias;                       // biased exponent of a
+  if (E <= 0) {                               // subnormal: quantum 2^(1-bias-mant)
+    double q = ldexp(a, mant + bias - 1);
+    return (uint32_t)nearbyint(q);            // may round up into the smallest normal
+  }
+  double q = ldexp(a, mant - (e - 1));        // in [2^mant, 2^(mant+1))
+  double r = nearbyint(q);
+  if (r == ldexp(1.0, mant + 1)) { r = ldexp(1.0, mant); ++E; }
+  if (E >= emax) { *ovf = 1; return (uint32_t)emax << mant; }
+  return ((uint32_t)E << mant) | ((uint32_t)r - (1u << mant));
+}
+
+static uint16_t to_bf16(double d, int *ovf) {
+  uint16_t sign = signbit(d) ? 0x8000 : 0;
+  double a = fabs(d);
+  if (isnan(a)) return sign | 0x7fc0;
+  if (isinf(a)) { *ovf = 1; return sign | 0x7f80; }
+  return sign | (uint16_t)round_small(a, 7, 127, 255, ovf);
+}
+
+static uint16_t to_f16(double d, int *ovf) {
+  uint16_t sign = signbit(d) ? 0x8000 : 0;
+  double a = fabs(d);
+  if (isnan(a)) return sign | 0x7e00;
+  if (isinf(a)) { *ovf = 1; return sign | 0x7c00; }
+  return sign | (uint16_t)round_small(a, 10, 15, 31, ovf);
+}
+
+static size_t storage_bytes(einspline_storage s) {
+  switch (s) {
+    case EINSPLINE_STORE_F32: return 4;
+    case EINSPLINE_STORE_F64: return 8;
+    case EINSPLINE_STORE_BF16:
+    case EINSPLINE_STORE_F16: return 2;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// device tables
+
+namespace {
+
+struct TableExt {
+  multi_UBspline_3d_hip pub;   // must stay first
+  int lane_bytes;              // tile: bytes per lane per row
+  int block;                   // tile: work-items per block
+};
+
+}  // namespace
+
+// Upload `src` (host table: Nx*Ny*Nz rows of `nreal` reals, float if
+// src_float else double, row stride `src_row` reals) into a new device
+// table of storage `st`.
+static multi_UBspline_3d_hip *upload(const void *src, int src_float, size_t src_row,
+                                     int Nx, int Ny, int Nz, int nsplines, int ncomp,
+                                     const Ugrid &xg, const Ugrid &yg, const Ugrid &zg,
+                                     type_code tcode, einspline_storage st) {
+  const size_t eb = storage_bytes(st);
+  if (!eb) { fail(EINSPLINE_EINVAL, "unknown storage type %d", (int)st); return nullptr; }
+  const size_t nreal = (size_t)nsplines * ncomp;
+  // pad rows to 32 bytes so every supported lane width divides them
+  const size_t per32 = 32 / eb;
+  const size_t row = (nreal + per32 - 1) / per32 * per32;
+  const size_t nrows = (size_t)Nx * Ny * Nz;
+  const size_t bytes = nrows * row * eb;
+
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { hip_fail(e, "hipGetDevice"); return nullptr; }
+  void *d = nullptr;
+  e = hipMalloc(&d, bytes);
+  if (e != hipSuccess) {
+    fail(EINSPLINE_ENOMEM, "hipMalloc of %zu bytes for a spline table: %s", bytes,
+         hipGetErrorString(e));
+    return nullptr;
+  }
+  // convert in pieces through a pinned staging buffer
+  const size_t stage_bytes = (size_t)64 << 20;
+  size_t rows_per = std::max<size_t>(1, stage_bytes / (row * eb));
+  rows_per = std::min(rows_per, nrows);
+  void *stage = nullptr;
+  e = hipHostMalloc(&stage, rows_per * row * eb, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    hipFree(d);
+    fail(EINSPLINE_ENOMEM, "hipHostMalloc staging: %s", hipGetErrorString(e));
+    return nullptr;
+  }
+  int ovf = 0;
+  for (size_t r0 = 0; r0 < nrows; r0 += rows_per) {
+    const size_t nr = std::min(rows_per, nrows - r0);
+    for (long rr = 0; rr < (long)nr; ++rr) {
+      const size_t r = r0 + rr;
+      unsigned char *dst = (unsigned char *)stage + (size_t)rr * row * eb;
+      for (size_t q = 0; q < row; ++q) {
+        double v = 0.0;
+        if (q < nreal)
+          v = src_float ? (double)((const float *)src)[r * src_row + q]
+                        : ((const double *)src)[r * src_row + q];
+        switch (st) {
+          case EINSPLINE_STORE_F32: ((float *)dst)[q] = (float)v; break;
+          case EINSPLINE_STORE_F64: ((double *)dst)[q] = v; break;
+          case EINSPLINE_STORE_BF16: ((uint16_t *)dst)[q] = to_bf16(v, &ovf); break;
+          case EINSPLINE_STORE_F16: ((uint16_t *)dst)[q] = to_f16(v, &ovf); break;
+        }
+      }
+    }
+    if (ovf) break;
+    e = hipMemcpy((unsigned char *)d + r0 * row * eb, stage, nr * row * eb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) break;
+  }
+  hipHostFree(stage);
+  if (ovf) {
+    hipFree(d);
+    fail(EINSPLINE_ERANGE, "coefficient outside the range of the storage type");
+    return nullptr;
+  }
+  if (e != hipSuccess) {
+    hipFree(d);
+    hip_fail(e, "hipMemcpy table upload");
+    return nullptr;
+  }
+  TableExt *t = (TableExt *)calloc(1, sizeof(TableExt));
+  if (!t) { hipFree(d); fail(EINSPLINE_ENOMEM, "out of host memory"); return nullptr; }
+  multi_UBspline_3d_hip *p = &t->pub;
+  p->coefs = d;
+  p->z_stride = (intptr_t)row;
+  p->y_stride = (intptr_t)Nz * row;
+  p->x_stride = (intptr_t)Ny * Nz * row;
+  p->x_grid = xg; p->y_grid = yg; p->z_grid = zg;
+  p->Nx = Nx; p->Ny = Ny; p->Nz = Nz;
+  p->num_splines = nsplines;
+  p->row_reals = (int)row;
+  p->tcode = tcode;
+  p->storage = st;
+  p->bytes = bytes;
+  p->device = dev;
+  t->lane_bytes = 0;
+  t->block = 0;
+  return p;
+}
+
+template <class H>
+static void dims_of(const H *s, int &Nx, int &Ny, int &Nz) {
+  // z_stride == num_splines for tables from create_multi_UBspline_3d_*
+  Nz = (int)(s->y_stride / s->z_stride);
+  Ny = (int)(s->x_stride / s->y_stride);
+  const einspline_host3d *h = einspline_host_of((void *)s);
+  if (h) Nx = h->Nx;
+  else Nx = s->x_grid.num + ((s->xBC.lCode == PERIODIC || s->xBC.lCode == ANTIPERIODIC) ? 3 : 2);
+}
+
+#define UPLOAD_FROM(S, IS_FLOAT, NCOMP, TC, ST)                                       \
+  do {                                                                                \
+    if (!S || !S->coefs) { fail(EINSPLINE_EINVAL, "NULL spline"); return nullptr; }   \
+    int Nx, Ny, Nz;                                                                   \
+    dims_of(S, Nx, Ny, Nz);                                                           \
+    return upload(S->coefs, IS_FLOAT, (size_t)S->z_stride * NCOMP, Nx, Ny, Nz,        \
+                  S->num_splines, NCOMP, S->x_grid, S->y_grid, S->z_grid, TC, ST);    \
+  } while (0)
+
+extern "C" multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip(multi_UBspline_3d_s *s) {
+  UPLOAD_FROM(s, 1, 1, SINGLE_REAL, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip_conv(multi_UBspline_3d_d *s) {
+  UPLOAD_FROM(s, 0, 1, SINGLE_REAL, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip_store(const multi_UBspline_3d_d *s,
+                                                                        einspline_storage st) {
+  if (st == EINSPLINE_STORE_F64) { fail(EINSPLINE_EINVAL, "use create_multi_UBspline_3d_d_hip for F64"); return nullptr; }
+  UPLOAD_FROM(s, 0, 1, SINGLE_REAL, st);
+}
+extern "C" multi_UBspline_3d_d_hip *create_multi_UBspline_3d_d_hip(multi_UBspline_3d_d *s) {
+  UPLOAD_FROM(s, 0, 1, DOUBLE_REAL, EINSPLINE_STORE_F64);
+}
+extern "C" multi_UBspline_3d_c_hip *create_multi_UBspline_3d_c_hip(multi_UBspline_3d_c *s) {
+  UPLOAD_FROM(s, 1, 2, SINGLE_COMPLEX, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_3d_c_hip *create_multi_UBspline_3d_c_hip_conv(multi_UBspline_3d_z *s) {
+  UPLOAD_FROM(s, 0, 2, SINGLE_COMPLEX, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_3d_z_hip *create_multi_UBspline_3d_z_hip(multi_UBspline_3d_z *s) {
+  UPLOAD_FROM(s, 0, 2, DOUBLE_COMPLEX, EINSPLINE_STORE_F64);
+}
+
+extern "C" void destroy_multi_UBspline_3d_hip(multi_UBspline_3d_hip *t) {
+  if (!t) return;
+  if (t->coefs) hipFree(t->coefs);
+  free((TableExt *)t);
+}
+
+extern "C" void einspline_mirror_release(void *mirror) {
+  destroy_multi_UBspline_3d_hip((multi_UBspline_3d_hip *)mirror);
+}
+
+extern "C" int einspline_set_tile(multi_UBspline_3d_hip *t, int lane_bytes, int block) {
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
+  const bool wide = t->storage == EINSPLINE_STORE_F64;
+  if (lane_bytes && lane_bytes != 8 && lane_bytes != 16 && lane_bytes != 32)
+    return fail(EINSPLINE_EINVAL, "lane_bytes must be 8, 16 or 32");
+  if (wide && lane_bytes == 8) return fail(EINSPLINE_EINVAL, "F64 tables need lane_bytes >= 16");
+  if (block && (block < 64 || block > 256 || block % 64))
+    return fail(EINSPLINE_EINVAL, "block_threads must be 64, 128, 192 or 256");
+  TableExt *x = (TableExt *)t;
+  x->lane_bytes = lane_bytes;
+  x->block = block;
+  return EINSPLINE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launching
+
+static int result_reals(const multi_UBspline_3d_hip *t) {
+  return t->num_splines * ((t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX) ? 2 : 1);
+}
+
+// Fill the table-dependent kernel arguments and the tile.
+static int prepare(const multi_UBspline_3d_hip *t, int num, Launch &L, Params &P) {
+  if (!t || !t->coefs) return fail(EINSPLINE_EINVAL, "NULL device table");
+  if (num < 0) return fail(EINSPLINE_EINVAL, "negative point count");
+  const TableExt *x = (const TableExt *)t;
+  const size_t eb = storage_bytes(t->storage);
+  int lb = x->lane_bytes ? x->lane_bytes : 16;
+  const int row_bytes = t->row_reals * (int)eb;
+  // narrow rows: use the widest lane that still gives every lane work
+  while (lb > 8 && row_bytes % lb) lb /= 2;
+  if (t->storage == EINSPLINE_STORE_F64 && lb < 16) lb = 16;
+  L.lane_bytes = lb;
+  L.block = x->block ? x->block : 256;
+  L.cplx = t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX;
+  memset(&P, 0, sizeof P);
+  P.coefs = t->coefs;
+  P.xs = t->x_stride; P.ys = t->y_stride; P.zs = t->z_stride;
+  P.x0 = t->x_grid.start; P.y0 = t->y_grid.start; P.z0 = t->z_grid.start;
+  P.xi = t->x_grid.delta_inv; P.yi = t->y_grid.delta_inv; P.zi = t->z_grid.delta_inv;
+  P.nx4 = t->Nx - 4; P.ny4 = t->Ny - 4; P.nz4 = t->Nz - 4;
+  P.chunks = row_bytes / lb;
+  P.nreal = result_reals(t);
+  P.npoints = num;
+  if ((long long)num * P.chunks >= (1ll << 31))
+    return fail(EINSPLINE_EINVAL, "batch too large: points * lanes per point must be < 2^31");
+  return EINSPLINE_OK;
+}
+
+static hipError_t dispatch(const multi_UBspline_3d_hip *t, const Launch &L, const Params &P,
+                           hipStream_t s) {
+  if (P.npoints == 0) return hipSuccess;
+  switch (t->storage) {
+    case EINSPLINE_STORE_F32: return esk::launch_f32(L, P, s);
+    case EINSPLINE_STORE_F64: return esk::launch_f64(L, P, s);
+    case EINSPLINE_STORE_BF16: return esk::launch_bf16(L, P, s);
+    case EINSPLINE_STORE_F16: return esk::launch_f16(L, P, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
+                       void *v, void *g, void *h, int packed, void *stream) {
+  Launch L;
+  Params P;
+  int rc = prepare(t, num, L, P);
+  if (rc) return rc;
+  if (num == 0) return EINSPLINE_OK;
+  if (!pos || !v || (mode >= esk::MODE_VG && !g) || (mode >= esk::MODE_VGL && !h))
+    return fail(EINSPLINE_EINVAL, "NULL position or output pointer");
+  L.mode = mode;
+  L.api = esk::API_STRIDED;
+  P.pos = pos;
+  P.out_v = v; P.out_g = g; P.out_h = h;
+  P.ld_v = P.nreal;
+  P.ld_g = 3l * P.nreal;
+  P.ld_h = mode == esk::MODE_VGL ? (long)P.nreal : (packed ? 6l : 9l) * P.nreal;
+  P.hpacked = packed ? 1 : 0;
+  hipError_t e = dispatch(t, L, P, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_eval_v_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
+                                      void *v, void *stream) {
+  return run_strided(t, esk::MODE_V, pos, num, v, nullptr, nullptr, 0, stream);
+}
+extern "C" int einspline_eval_vg_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
+                                       void *v, void *g, void *stream) {
+  return run_strided(t, esk::MODE_VG, pos, num, v, g, nullptr, 0, stream);
+}
+extern "C" int einspline_eval_vgl_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
+                                        void *v, void *g, void *l, void *stream) {
+  return run_strided(t, esk::MODE_VGL, pos, num, v, g, l, 0, stream);
+}
+extern "C" int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
+                                        void *v, void *g, void *h, int packed, void *stream) {
+  return run_strided(t, esk::MODE_VGH, pos, num, v, g, h, packed, stream);
+}
+
+// CUDA-style: per-point pointer arrays, packed Hessian, synchronous.
+static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, const void *pos,
+                    bool pos_f32, void **v, void **g, void **h, int num) {
+  if (t && t->tcode != want)
+    return fail(EINSPLINE_EINVAL, "device table has the wrong type for this entry point");
+  if (t && want == SINGLE_REAL && t->storage == EINSPLINE_STORE_F64)
+    return fail(EINSPLINE_EINVAL, "F64 storage in a single-precision table");
+  Launch L;
+  Params P;
+  int rc = prepare(t, num, L, P);
+  if (rc) return rc;
+  if (num == 0) return EINSPLINE_OK;
+  if (!pos || !v || (mode == esk::MODE_VGH && (!g || !h)))
+    return fail(EINSPLINE_EINVAL, "NULL position or output pointer array");
+  L.lane_bytes = 16;            // the pointer-array kernels exist for 16-byte lanes
+  if (t->row_reals * (int)storage_bytes(t->storage) % 16)
+    return fail(EINSPLINE_EINVAL, "row not a multiple of 16 bytes");
+  P.chunks = t->row_reals * (int)storage_bytes(t->storage) / 16;
+  L.mode = mode;
+  L.api = pos_f32 ? esk::API_PTRS_F32POS : esk::API_PTRS_F64POS;
+  P.pos = pos;
+  P.out_v = v; P.out_g = g; P.out_h = h;
+  P.hpacked = 1;
+  hipError_t e = dispatch(t, L, P, nullptr);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+  return EINSPLINE_OK;
+}
+
+extern "C" int eval_multi_multi_UBspline_3d_s_hip(multi_UBspline_3d_s_hip *t, float *pos,
+                                                  float *vals[], int num) {
+  return run_ptrs(t, SINGLE_REAL, esk::MODE_V, pos, true, (void **)vals, nullptr, nullptr, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_s_vgh_hip(multi_UBspline_3d_s_hip *t, float *pos,
+                                                      float *vals[], float *grads[],
+                                                      float *hess[], int num) {
+  return run_ptrs(t, SINGLE_REAL, esk::MODE_VGH, pos, true, (void **)vals, (void **)grads,
+                  (void **)hess, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_d_hip(multi_UBspline_3d_d_hip *t, double *pos,
+                                                  double *vals[], int num) {
+  return run_ptrs(t, DOUBLE_REAL, esk::MODE_V, pos, false, (void **)vals, nullptr, nullptr, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_d_vgh_hip(multi_UBspline_3d_d_hip *t, double *pos,
+                                                      double *vals[], double *grads[],
+                                                      double *hess[], int num) {
+  return run_ptrs(t, DOUBLE_REAL, esk::MODE_VGH, pos, false, (void **)vals, (void **)grads,
+                  (void **)hess, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_c_hip(multi_UBspline_3d_c_hip *t, float *pos,
+                                                  complex_float *vals[], int num) {
+  return run_ptrs(t, SINGLE_COMPLEX, esk::MODE_V, pos, true, (void **)vals, nullptr, nullptr, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_c_vgh_hip(multi_UBspline_3d_c_hip *t, float *pos,
+                                                      complex_float *vals[],
+                                                      complex_float *grads[],
+                                                      complex_float *hess[], int num) {
+  return run_ptrs(t, SINGLE_COMPLEX, esk::MODE_VGH, pos, true, (void **)vals, (void **)grads,
+                  (void **)hess, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_z_hip(multi_UBspline_3d_z_hip *t, double *pos,
+                                                  complex_double *vals[], int num) {
+  return run_ptrs(t, DOUBLE_COMPLEX, esk::MODE_V, pos, false, (void **)vals, nullptr, nullptr, num);
+}
+extern "C" int eval_multi_multi_UBspline_3d_z_vgh_hip(multi_UBspline_3d_z_hip *t, double *pos,
+                                                      complex_double *vals[],
+                                                      complex_double *grads[],
+                                                      complex_double *hess[], int num) {
+  return run_ptrs(t, DOUBLE_COMPLEX, esk::MODE_VGH, pos, false, (void **)vals, (void **)grads,
+                  (void **)hess, num);
+}
+
+// ---------------------------------------------------------------------------
+// single-point path: einspline's original API, one point per call
+
+namespace {
+
+// Per-thread staging: device position + outputs, pinned host copy, stream.
+// Deliberately not freed at thread exit: the HIP runtime may already be
+// shut down when thread-local destructors of the main thread run.
+struct Staging {
+  void *dev = nullptr;
+  void *host = nullptr;
+  size_t bytes = 0;
+  hipStream_t stream = nullptr;
+  int device = -1;
+};
+thread_local Staging g_stage;
+
+[[noreturn]] void die(const char *fn) {
+  fprintf(stderr, "%s: %s\n", fn, g_err);
+  abort();
+}
+
+int stage_reserve(size_t bytes) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (g_stage.device != dev) {
+    if (g_stage.dev) hipFree(g_stage.dev);
+    if (g_stage.host) hipHostFree(g_stage.host);
+    if (g_stage.stream) hipStreamDestroy(g_stage.stream);
+    g_stage.dev = g_stage.host = nullptr;
+    g_stage.bytes = 0;
+    g_stage.stream = nullptr;
+    g_stage.device = dev;
+  }
+  if (!g_stage.stream) {
+    e = hipStreamCreateWithFlags(&g_stage.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  }
+  if (g_stage.bytes >= bytes) return EINSPLINE_OK;
+  if (g_stage.dev) hipFree(g_stage.dev);
+  if (g_stage.host) hipHostFree(g_stage.host);
+  g_stage.dev = g_stage.host = nullptr;
+  g_stage.bytes = 0;
+  e = hipMalloc(&g_stage.dev, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc staging");
+  e = hipHostMalloc(&g_stage.host, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return hip_fail(e, "hipHostMalloc staging");
+  g_stage.bytes = bytes;
+  return EINSPLINE_OK;
+}
+
+// Device mirror of a host table, rebuilt when set_* or the caller
+// invalidated it.
+const multi_UBspline_3d_hip *mirror_of(void *spline, const char *fn) {
+  einspline_host3d *h = einspline_host_of(spline);
+  if (!h) { fail(EINSPLINE_EINVAL, "spline was not created by this library"); die(fn); }
+  pthread_mutex_lock(&h->lock);
+  if (h->dirty || !h->mirror) {
+    if (h->mirror) destroy_multi_UBspline_3d_hip((multi_UBspline_3d_hip *)h->mirror);
+    h->mirror = nullptr;
+    multi_UBspline_3d_hip *m = nullptr;
+    switch (h->pub.s.tcode) {
+      case SINGLE_REAL: m = create_multi_UBspline_3d_s_hip(&h->pub.s); break;
+      case DOUBLE_REAL: m = create_multi_UBspline_3d_d_hip(&h->pub.d); break;
+      case SINGLE_COMPLEX: m = create_multi_UBspline_3d_c_hip(&h->pub.c); break;
+      case DOUBLE_COMPLEX: m = create_multi_UBspline_3d_z_hip(&h->pub.z); break;
+    }
+    if (!m) { pthread_mutex_unlock(&h->lock); die(fn); }
+    h->mirror = m;
+    h->dirty = 0;
+  }
+  const multi_UBspline_3d_hip *m = (const multi_UBspline_3d_hip *)h->mirror;
+  pthread_mutex_unlock(&h->lock);
+  return m;
+}
+
+// Evaluate one point and copy `ncomp` result blocks of `elt`-byte result
+// elements back: values (N), gradients (3N), Laplacian (N) or Hessian (9N).
+void eval_point(void *spline, const char *fn, int mode, double x, double y, double z,
+                void *vals, void *grads, void *lh) {
+  const multi_UBspline_3d_hip *t = mirror_of(spline, fn);
+  const size_t elt = (t->storage == EINSPLINE_STORE_F64 ? 8 : 4) *
+                     ((t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX) ? 2 : 1);
+  const size_t N = (size_t)t->num_splines;
+  const size_t nv = N * elt, ng = mode >= esk::MODE_VG ? 3 * N * elt : 0;
+  const size_t nh = mode == esk::MODE_VGH ? 9 * N * elt : mode == esk::MODE_VGL ? N * elt : 0;
+  // layout in the staging buffer: [pos | vals | grads | lapl/hess], 256-byte aligned parts
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_v = 256, o_g = o_v + up(nv), o_h = o_g + up(ng), total = o_h + up(nh);
+  if (stage_reserve(total)) die(fn);
+  unsigned char *hb = (unsigned char *)g_stage.host, *db = (unsigned char *)g_stage.dev;
+  double *hp = (double *)hb;
+  hp[0] = x; hp[1] = y; hp[2] = z;
+  hipError_t e = hipMemcpyAsync(db, hb, 3 * sizeof(double), hipMemcpyHostToDevice, g_stage.stream);
+  if (e != hipSuccess) { hip_fail(e, "hipMemcpyAsync"); die(fn); }
+  int rc = run_strided(t, mode, (const double *)db, 1, db + o_v, ng ? db + o_g : nullptr,
+                       nh ? db + o_h : nullptr, 0, g_stage.stream);
+  if (rc) die(fn);
+  e = hipMemcpyAsync(hb + o_v, db + o_v, total - o_v, hipMemcpyDeviceToHost, g_stage.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stage.stream);
+  if (e != hipSuccess) { hip_fail(e, "result copy"); die(fn); }
+  memcpy(vals, hb + o_v, nv);
+  if (ng) memcpy(grads, hb + o_g, ng);
+  if (nh) memcpy(lh, hb + o_h, nh);
+}
+
+}  // namespace
+
+#define POINT_API(T, TY)                                                                       \
+  extern "C" void eval_multi_UBspline_3d_##T(multi_UBspline_3d_##T *s, double x, double y,    \
+                                             double z, TY *v) {                                \
+    eval_point(s, "eval_multi_UBspline_3d_" #T, esk::MODE_V, x, y, z, v, nullptr, nullptr);   \
+  }                                                                                            \
+  extern "C" void eval_multi_UBspline_3d_##T##_vg(multi_UBspline_3d_##T *s, double x,         \
+                                                  double y, double z, TY *v, TY *g) {          \
+    eval_point(s, "eval_multi_UBspline_3d_" #T "_vg", esk::MODE_VG, x, y, z, v, g, nullptr);  \
+  }                                                                                            \
+  extern "C" void eval_multi_UBspline_3d_##T##_vgl(multi_UBspline_3d_##T *s, double x,        \
+                                                   double y, double z, TY *v, TY *g, TY *l) {  \
+    eval_point(s, "eval_multi_UBspline_3d_" #T "_vgl", esk::MODE_VGL, x, y, z, v, g, l);      \
+  }                                                                                            \
+  extern "C" void eval_multi_UBspline_3d_##T##_vgh(multi_UBspline_3d_##T *s, double x,        \
+                                                   double y, double z, TY *v, TY *g, TY *h) {  \
+    eval_point(s, "eval_multi_UBspline_3d_" #T "_vgh", esk::MODE_VGH, x, y, z, v, g, h);      \
+  }
+
+POINT_API(s, float)
+POINT_API(d, double)
+POINT_API(c, complex_float)
+POINT_API(z, complex_double)
+
+// ---------------------------------------------------------------------------
+// minimal device runtime
+
+extern "C" int einspline_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+extern "C" void *einspline_hip_malloc(size_t bytes) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess) { hip_fail(e, "hipMalloc"); return nullptr; }
+  return p;
+}
+extern "C" int einspline_hip_free(void *p) {
+  hipError_t e = hipFree(p);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipFree");
+}
+extern "C" int einspline_hip_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipMemcpy H2D");
+}
+extern "C" int einspline_hip_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipMemcpy D2H");
+}
+extern "C" int einspline_hip_memset(void *p, int v, size_t bytes) {
+  hipError_t e = hipMemset(p, v, bytes);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipMemset");
+}
+extern "C" int einspline_hip_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? 0 : hip_fail(e, "hipDeviceSynchronize");
+}
+extern "C" void *einspline_hip_stream_create(void) {
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) { hip_fail(e, "hipStreamCreate"); return nullptr; }
+  return s;
+}
+extern "C" int einspline_hip_stream_destroy(void *s) {
+  hipError_t e = hipStreamDestroy((hipStream_t)s);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipStreamDestroy");
+}
+extern "C" int einspline_hip_stream_synchronize(void *s) {
+  hipError_t e = hipStreamSynchronize((hipStream_t)s);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipStreamSynchronize");
+}
+extern "C" void *einspline_hip_event_create(void) {
+  hipEvent_t ev = nullptr;
+  hipError_t e = hipEventCreate(&ev);
+  if (e != hipSuccess) { hip_fail(e, "hipEventCreate"); return nullptr; }
+  return ev;
+}
+extern "C" int einspline_hip_event_destroy(void *ev) {
+  hipError_t e = hipEventDestroy((hipEvent_t)ev);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipEventDestroy");
+}
+extern "C" int einspline_hip_event_record(void *ev, void *s) {
+  hipError_t e = hipEventRecord((hipEvent_t)ev, (hipStream_t)s);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipEventRecord");
+}
+extern "C" int einspline_hip_event_synchronize(void *ev) {
+  hipError_t e = hipEventSynchronize((hipEvent_t)ev);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipEventSynchronize");
+}
+extern "C" float einspline_hip_event_elapsed_ms(void *a, void *b) {
+  float ms = -1.0f;
+  hipError_t e = hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b);
+  if (e != hipSuccess) { hip_fail(e, "hipEventElapsedTime"); return -1.0f; }
+  return ms;
+}

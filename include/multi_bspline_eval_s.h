@@ -1,0 +1,52 @@
+/*
+ * multi_bspline_eval_s.h -- single-point evaluation of 3-D uniform
+ * single-precision real multi-splines.
+ *
+ * Replaces: src/multi_bspline_eval_s.h:86-109 (3-D prototypes).
+ *
+ * Each call evaluates all num_splines orbitals at one point (x,y,z) and
+ * writes vals[N], grads[3N] (gradient of orbital n at 3n..3n+2), lapl[N]
+ * (Laplacian) or hess[9N] (row-major symmetric 3x3 Hessian of orbital n at
+ * 9n..9n+8).  Output pointers are host memory.  The work runs on the GPU:
+ * the first call after a set_* (or einspline_hip_invalidate) uploads the
+ * table, then each call is one kernel launch plus a copy of the results.
+ *
+ * Points must lie in [start, end] on every axis, as in the reference;
+ * points outside are evaluated by extending the edge cell's polynomial
+ * (memory-safe, otherwise unspecified).  Errors print to stderr and abort,
+ * as the reference's GPU path does.
+ *
+ * The reference declares these without extern "C" and builds them twice
+ * (C and C++ symbols); here they are C symbols for both languages.
+ */
+#ifndef EINSPLINE_MULTI_BSPLINE_EVAL_S_H
+#define EINSPLINE_MULTI_BSPLINE_EVAL_S_H
+
+#include "bspline_base.h"
+#include "multi_bspline_structs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void eval_multi_UBspline_3d_s(multi_UBspline_3d_s *spline,
+                               double x, double y, double z,
+                               float *vals);
+
+void eval_multi_UBspline_3d_s_vg(multi_UBspline_3d_s *spline,
+                                  double x, double y, double z,
+                                  float *vals, float *grads);
+
+void eval_multi_UBspline_3d_s_vgl(multi_UBspline_3d_s *spline,
+                                   double x, double y, double z,
+                                   float *vals, float *grads, float *lapl);
+
+void eval_multi_UBspline_3d_s_vgh(multi_UBspline_3d_s *spline,
+                                   double x, double y, double z,
+                                   float *vals, float *grads, float *hess);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,180 @@
+/*
+ * multi_bspline_hip.h -- device-resident 3-D multi-spline tables and
+ * batched many-point evaluation on AMD Instinct MI355X (gfx950).
+ *
+ * Replaces the reference's GPU path:
+ *   src/multi_bspline_create_cuda.h:60-78  create_multi_UBspline_3d_*_cuda{,_conv}
+ *   src/multi_bspline_eval_cuda.h:62-137   eval_multi_multi_UBspline_3d_*{,_vgh,_vgl}_cuda
+ *   src/multi_bspline_structs_cuda.h       multi_UBspline_3d_*_cuda
+ * The `_hip` functions keep the reference's argument meaning (a device table
+ * built from a host spline, `num` points per call, device arrays of
+ * per-point output pointers).  The `_batch` functions are this library's
+ * main entry points: outputs are strided device arrays, positions are
+ * doubles, work is enqueued on a caller-chosen stream and nothing
+ * synchronises.
+ *
+ * All pointers named *_d are device memory.  `stream` is a hipStream_t
+ * (NULL = the null stream); it is typed void* so that this header needs no
+ * HIP headers.  Functions returning int give 0 on success and a negative
+ * EINSPLINE_E* code on failure; einspline_hip_last_error() describes the
+ * most recent failure of the calling thread.
+ */
+#ifndef EINSPLINE_MULTI_BSPLINE_HIP_H
+#define EINSPLINE_MULTI_BSPLINE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "bspline_base.h"
+#include "multi_bspline_structs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define EINSPLINE_OK          0
+#define EINSPLINE_EINVAL     -1   /* bad argument (NULL, size, type mismatch) */
+#define EINSPLINE_ENOMEM     -2   /* device or host allocation failed */
+#define EINSPLINE_EHIP       -3   /* HIP runtime error (launch, copy, no device) */
+#define EINSPLINE_ERANGE     -4   /* value not representable in the storage type */
+#define EINSPLINE_EBUDGET    -5   /* serving: table larger than the HBM budget */
+#define EINSPLINE_EBUSY      -6   /* serving: no free slot / queue full */
+
+/* ---- device tables ---------------------------------------------------- */
+
+/* Element type of a device table's coefficients.  Evaluation accumulates in
+ * double for F64 tables and in float otherwise. */
+typedef enum {
+  EINSPLINE_STORE_F32  = 0,
+  EINSPLINE_STORE_F64  = 1,
+  EINSPLINE_STORE_BF16 = 2,
+  EINSPLINE_STORE_F16  = 3
+} einspline_storage;
+
+/* Device copy of a multi_UBspline_3d_* table.  Coefficients are laid out as
+ * in the host struct, coefs[ix][iy][iz][r], with the row padded with zeros
+ * to `row_reals` reals (r counts reals: a complex orbital n owns reals 2n and
+ * 2n+1).  Strides are in reals. */
+typedef struct multi_UBspline_3d_hip {
+  void     *coefs;                       /* device memory, owned */
+  intptr_t  x_stride, y_stride, z_stride;
+  Ugrid     x_grid, y_grid, z_grid;
+  int       Nx, Ny, Nz;                  /* table extent incl. the 3 (or 2) halo planes */
+  int       num_splines;                 /* orbitals; a complex orbital counts once */
+  int       row_reals;                   /* == z_stride */
+  type_code tcode;                       /* result type of evaluation */
+  einspline_storage storage;
+  size_t    bytes;                       /* device bytes of coefs */
+  int       device;                      /* HIP device ordinal */
+} multi_UBspline_3d_hip;
+
+typedef multi_UBspline_3d_hip multi_UBspline_3d_s_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_3d_d_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_3d_c_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_3d_z_hip;
+
+/* Upload a host table to the current device (src/multi_bspline_create_cuda.h:60-78).
+ * Unlike the reference, any boundary condition is accepted and grid.start
+ * is honoured.  Returns NULL on failure (see einspline_hip_last_error). */
+multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip(multi_UBspline_3d_s *spline);
+multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip_conv(multi_UBspline_3d_d *spline);
+multi_UBspline_3d_d_hip *create_multi_UBspline_3d_d_hip(multi_UBspline_3d_d *spline);
+multi_UBspline_3d_c_hip *create_multi_UBspline_3d_c_hip(multi_UBspline_3d_c *spline);
+multi_UBspline_3d_c_hip *create_multi_UBspline_3d_c_hip_conv(multi_UBspline_3d_z *spline);
+multi_UBspline_3d_z_hip *create_multi_UBspline_3d_z_hip(multi_UBspline_3d_z *spline);
+
+/* Precision form (no reference counterpart): a table solved in double and
+ * stored as F32, BF16 or F16 (round to nearest even); evaluation returns
+ * float results with float accumulation.  F16 fails with EINSPLINE_ERANGE
+ * if a coefficient exceeds the half-precision range. */
+multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip_store(const multi_UBspline_3d_d *spline,
+                                                             einspline_storage storage);
+
+/* Free a device table (the reference has none; its callers cudaFree by hand). */
+void destroy_multi_UBspline_3d_hip(multi_UBspline_3d_hip *table);
+
+/* ---- CUDA-style batched evaluation -------------------------------------
+ * src/multi_bspline_eval_cuda.h:62-137.  pos_d[3*i+{0,1,2}] is point i;
+ * vals_d[i], grads_d[i], hess_d[i] are device pointers (held in device
+ * arrays) to point i's outputs.  Gradients are [n][3]; Hessians are packed
+ * as 6 unique entries [xx,xy,xz,yy,yz,zz] per orbital, as the reference's
+ * CUDA kernels write them; complex results interleave re/im per entry.
+ * These calls synchronise the device before returning, as the reference's
+ * do. */
+int eval_multi_multi_UBspline_3d_s_hip(multi_UBspline_3d_s_hip *spline, float *pos_d,
+                                       float *vals_d[], int num);
+int eval_multi_multi_UBspline_3d_s_vgh_hip(multi_UBspline_3d_s_hip *spline, float *pos_d,
+                                           float *vals_d[], float *grads_d[], float *hess_d[],
+                                           int num);
+int eval_multi_multi_UBspline_3d_d_hip(multi_UBspline_3d_d_hip *spline, double *pos_d,
+                                       double *vals_d[], int num);
+int eval_multi_multi_UBspline_3d_d_vgh_hip(multi_UBspline_3d_d_hip *spline, double *pos_d,
+                                           double *vals_d[], double *grads_d[], double *hess_d[],
+                                           int num);
+int eval_multi_multi_UBspline_3d_c_hip(multi_UBspline_3d_c_hip *spline, float *pos_d,
+                                       complex_float *vals_d[], int num);
+int eval_multi_multi_UBspline_3d_c_vgh_hip(multi_UBspline_3d_c_hip *spline, float *pos_d,
+                                           complex_float *vals_d[], complex_float *grads_d[],
+                                           complex_float *hess_d[], int num);
+int eval_multi_multi_UBspline_3d_z_hip(multi_UBspline_3d_z_hip *spline, double *pos_d,
+                                       complex_double *vals_d[], int num);
+int eval_multi_multi_UBspline_3d_z_vgh_hip(multi_UBspline_3d_z_hip *spline, double *pos_d,
+                                           complex_double *vals_d[], complex_double *grads_d[],
+                                           complex_double *hess_d[], int num);
+
+/* ---- strided batched evaluation (main entry points) --------------------
+ * Evaluates `num` points pos_d[3*i+{0,1,2}] (doubles, any storage type) in
+ * one launch on `stream`.  Point i's results go to
+ *   vals_d  + i*N          (N = num_splines values)
+ *   grads_d + i*3N         ([n][3])
+ *   lapl_d  + i*N          (vgl)
+ *   hess_d  + i*9N or i*6N (vgh; full row-major 3x3, or packed 6 if hess_packed)
+ * counted in result elements (float for F32/BF16/F16 tables, double for
+ * F64, complex for c/z).  Results are bit-for-bit independent of `num` and
+ * of the other points in the batch.  Does not synchronise. */
+int einspline_eval_v_batch(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
+                           void *vals_d, void *stream);
+int einspline_eval_vg_batch(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
+                            void *vals_d, void *grads_d, void *stream);
+int einspline_eval_vgl_batch(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
+                             void *vals_d, void *grads_d, void *lapl_d, void *stream);
+int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
+                             void *vals_d, void *grads_d, void *hess_d, int hess_packed,
+                             void *stream);
+
+/* Tile shape of the batched kernels: bytes of an orbital row one lane loads
+ * per stencil row (4, 8, 16 or 32) and work-items per block (64..256, a
+ * multiple of 64).  0 selects the built-in default.  Applies to subsequent
+ * launches on `spline`; returns EINSPLINE_EINVAL for unsupported shapes. */
+int einspline_set_tile(multi_UBspline_3d_hip *spline, int lane_bytes, int block_threads);
+
+/* ---- single-point path support ----------------------------------------
+ * The single-point eval_multi_UBspline_3d_* functions keep a device mirror
+ * of the host table and refresh it after set_*.  A caller that writes
+ * spline->coefs directly must call this before the next evaluation. */
+void einspline_hip_invalidate(void *spline);
+
+/* ---- minimal device runtime (so C and ctypes callers need no HIP) ------ */
+int         einspline_hip_device_count(void);
+const char *einspline_hip_last_error(void);
+void       *einspline_hip_malloc(size_t bytes);             /* NULL on failure */
+int         einspline_hip_free(void *ptr_d);
+int         einspline_hip_memcpy_htod(void *dst_d, const void *src, size_t bytes);
+int         einspline_hip_memcpy_dtoh(void *dst, const void *src_d, size_t bytes);
+int         einspline_hip_memset(void *ptr_d, int value, size_t bytes);
+int         einspline_hip_synchronize(void);
+void       *einspline_hip_stream_create(void);
+int         einspline_hip_stream_destroy(void *stream);
+int         einspline_hip_stream_synchronize(void *stream);
+void       *einspline_hip_event_create(void);
+int         einspline_hip_event_destroy(void *event);
+int         einspline_hip_event_record(void *event, void *stream);
+int         einspline_hip_event_synchronize(void *event);
+/* Milliseconds between two recorded events; negative on error. */
+float       einspline_hip_event_elapsed_ms(void *start, void *stop);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EINSPLINE_MULTI_BSPLINE_HIP_H */

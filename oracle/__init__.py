@@ -1,0 +1,149 @@
+"""oracle -- the correctness oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+import this package, and only as the checker (or the timed CPU baseline).
+The product path (einspline_amd) never touches it.
+
+* liboracle.so (oracle.c): C restatement of the reference's default-build
+  create/set and std evaluation (src/multi_bspline_create.c,
+  src/bspline_create.c, src/multi_bspline_eval_std_*_impl.h).
+* brute.py: numpy closed forms used to pin the restatement (textbook B3
+  kernel sums, FFT periodic solve, analytic plane waves).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+lib = C.CDLL(LIB_PATH)
+
+OR_PERIODIC, OR_DERIV1, OR_DERIV2, OR_FLAT, OR_NATURAL, OR_ANTIPERIODIC = range(6)
+MODE = {"v": 0, "vg": 1, "vgl": 2, "vgh": 3}
+
+
+class otable(C.Structure):
+    _fields_ = [("M", C.c_int * 3), ("Nn", C.c_int * 3),
+                ("start", C.c_double * 3), ("end", C.c_double * 3),
+                ("delta", C.c_double * 3), ("delta_inv", C.c_double * 3),
+                ("lcode", C.c_int * 3), ("rcode", C.c_int * 3),
+                ("lval", C.c_double * 6), ("rval", C.c_double * 6),
+                ("N", C.c_int), ("ncomp", C.c_int),
+                ("xs", C.c_long), ("ys", C.c_long), ("zs", C.c_long)]
+
+
+_vp, _i, _d = C.c_void_p, C.c_int, C.c_double
+lib.oracle_table_init.restype = _i
+lib.oracle_table_init.argtypes = [C.POINTER(otable), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i]
+for k in "sdcz":
+    f = getattr(lib, f"oracle_set_{k}")
+    f.restype = None
+    f.argtypes = [C.POINTER(otable), _vp, _i, _vp]
+for k in "sd":
+    f = getattr(lib, f"oracle_eval_{k}")
+    f.restype = _i
+    f.argtypes = [C.POINTER(otable), _vp, _i, _d, _d, _d, _vp, _vp, _vp]
+lib.oracle_eval_batch.restype = _i
+lib.oracle_eval_batch.argtypes = [C.POINTER(otable), _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _i]
+lib.oracle_basis_d.restype = None
+lib.oracle_basis_d.argtypes = [_d, _vp, _vp, _vp]
+lib.oracle_max_threads.restype = _i
+
+
+def basis(t: float):
+    a, da, d2a = (np.zeros(4) for _ in range(3))
+    lib.oracle_basis_d(float(t), a.ctypes.data, da.ctypes.data, d2a.ctypes.data)
+    return a, da, d2a
+
+
+def max_threads() -> int:
+    return lib.oracle_max_threads()
+
+
+class Table:
+    """Oracle spline table of kind s/d/c/z (numpy-owned coefficients)."""
+
+    def __init__(self, kind, M, start, end, N, bc=None):
+        self.kind = kind
+        self.ncomp = 2 if kind in "cz" else 1
+        self.real = np.float32 if kind in "sc" else np.float64
+        bc = bc or [(OR_PERIODIC, OR_PERIODIC)] * 3
+        lc = np.array([b[0] for b in bc], np.int32)
+        rc = np.array([b[1] for b in bc], np.int32)
+        lv = np.zeros(6)
+        rv = np.zeros(6)
+        for a, b in enumerate(bc):
+            if len(b) > 2:
+                lv[2 * a], lv[2 * a + 1] = np.real(b[2]), np.imag(b[2])
+                rv[2 * a], rv[2 * a + 1] = np.real(b[3]), np.imag(b[3])
+        self.t = otable()
+        # keep every argument array alive across the call
+        m_a = np.array(M, np.int32)
+        s_a = np.array(start, float)
+        e_a = np.array(end, float)
+        rc_ = lib.oracle_table_init(C.byref(self.t), m_a.ctypes.data, s_a.ctypes.data,
+                                    e_a.ctypes.data, lc.ctypes.data, rc.ctypes.data,
+                                    lv.ctypes.data, rv.ctypes.data, int(N), self.ncomp)
+        assert rc_ == 0, "invalid oracle table"
+        self.N = int(N)
+        self.M = tuple(M)
+        self.shape = tuple(self.t.Nn)
+        self.coefs = np.zeros(self.shape + (self.N * self.ncomp,), self.real)
+
+    @property
+    def delta_inv(self):
+        return np.array(self.t.delta_inv[:])
+
+    def set(self, n, data):
+        data = np.asarray(data)
+        if self.ncomp == 2:
+            cd = np.complex64 if self.kind == "c" else np.complex128
+            data = np.ascontiguousarray(data, dtype=cd)
+        else:
+            data = np.ascontiguousarray(data, dtype=self.real)
+        assert data.size == int(np.prod(self.M))
+        getattr(lib, f"oracle_set_{self.kind}")(C.byref(self.t), self.coefs.ctypes.data, int(n),
+                                                 data.ctypes.data)
+
+    def eval_batch(self, mode, pos, nthreads=0):
+        """pos: (P, 3) float64.  Returns (vals, grads, lapl/hess) as arrays of
+        shape (P, N), (P, N, 3), (P, N) or (P, N, 3, 3); complex for c/z."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        P = pos.shape[0]
+        NR = self.N * self.ncomp
+        m = MODE[mode]
+        v = np.zeros((P, NR), self.real)
+        g = np.zeros((P, 3 * NR), self.real) if m >= 1 else None
+        nh = 9 if mode == "vgh" else 1 if mode == "vgl" else 0
+        h = np.zeros((P, nh * NR), self.real) if nh else None
+        bad = lib.oracle_eval_batch(C.byref(self.t), 1 if self.real == np.float64 else 0,
+                                    self.coefs.ctypes.data, m, pos.ctypes.data, P, v.ctypes.data,
+                                    g.ctypes.data if g is not None else None,
+                                    h.ctypes.data if h is not None else None, int(nthreads))
+        self.bad = bad
+        return shape_results(mode, self.N, self.ncomp, v, g, h)
+
+
+def shape_results(mode, N, ncomp, v, g, h):
+    """Reshape flat per-point result blocks (CPU layout) to (P, N, ...)."""
+    P = v.shape[0]
+
+    def cx(a):
+        if ncomp == 2:
+            return a.view(np.complex64 if a.dtype == np.float32 else np.complex128)
+        return a
+
+    out = [cx(v).reshape(P, N)]
+    if g is not None:
+        out.append(cx(g).reshape(P, N, 3))
+    if h is not None:
+        if mode == "vgl":
+            out.append(cx(h).reshape(P, N))
+        else:
+            out.append(cx(h).reshape(P, N, 3, 3))
+    return tuple(out)

@@ -1,0 +1,42 @@
+/* oracle.h -- CPU restatement of einspline's 3-D multi-spline create/eval.
+ * TEST INFRASTRUCTURE ONLY (see oracle.c). */
+#ifndef EINSPLINE_ORACLE_H
+#define EINSPLINE_ORACLE_H
+#include <stddef.h>
+
+/* boundary codes, same values as bc_code in src/bspline_base.h:49 */
+enum { OR_PERIODIC = 0, OR_DERIV1, OR_DERIV2, OR_FLAT, OR_NATURAL, OR_ANTIPERIODIC };
+/* evaluation modes */
+enum { OR_V = 0, OR_VG = 1, OR_VGL = 2, OR_VGH = 3 };
+
+typedef struct {
+  int M[3];              /* data points per axis */
+  int Nn[3];             /* coefficients per axis */
+  double start[3], end[3], delta[3], delta_inv[3];
+  int lcode[3], rcode[3];
+  double lval[3][2], rval[3][2];   /* derivative-condition values (re, im) */
+  int N;                 /* orbitals */
+  int ncomp;             /* 1 real, 2 complex */
+  long xs, ys, zs;       /* strides in elements (complex element = ncomp reals) */
+} otable;
+
+int oracle_table_init(otable *t, const int M[3], const double start[3], const double end[3],
+                      const int lcode[3], const int rcode[3], const double lval[6],
+                      const double rval[6], int N, int ncomp);
+size_t oracle_table_elems(const otable *t);
+
+void oracle_set_s(const otable *t, float *coefs, int n, const float *data);
+void oracle_set_d(const otable *t, double *coefs, int n, const double *data);
+void oracle_set_c(const otable *t, float *coefs, int n, const float *data);
+void oracle_set_z(const otable *t, double *coefs, int n, const double *data);
+
+int oracle_eval_s(const otable *t, const float *coefs, int mode, double x, double y, double z,
+                  float *v, float *g, float *lh);
+int oracle_eval_d(const otable *t, const double *coefs, int mode, double x, double y, double z,
+                  double *v, double *g, double *lh);
+int oracle_eval_batch(const otable *t, int is_double, const void *coefs, int mode,
+                      const double *pos, int npts, void *vals, void *grads, void *lh,
+                      int nthreads);
+void oracle_basis_d(double t, double a[4], double da[4], double d2a[4]);
+int oracle_max_threads(void);
+#endif

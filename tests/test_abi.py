@@ -1,0 +1,108 @@
+"""The C-ABI boundary, checked without a GPU: the library loads, exports
+every function include/*.h declares, and the public structs have the
+reference's x86-64 layout (src/bspline_base.h:49-96,
+src/multi_bspline_structs.h:51-180)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+
+
+def declared_functions():
+    names = set()
+    for f in sorted(os.listdir(INC)):
+        if not f.endswith(".h"):
+            continue
+        src = open(os.path.join(INC, f)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"//[^\n]*", "", src)
+        src = re.sub(r"#[^\n]*", "", src)
+        for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\(", src):
+            name = m.group(1)
+            # a declaration: name( ... ) followed by ';' at top level
+            rest = src[m.end():]
+            depth, i = 1, 0
+            while depth and i < len(rest):
+                depth += {"(": 1, ")": -1}.get(rest[i], 0)
+                i += 1
+            if rest[i:].lstrip().startswith(";") and name not in ("sizeof",):
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_function():
+    import einspline_amd as ea  # noqa: F401  (raises if the .so is missing)
+    lib = os.path.join(ROOT, "einspline_amd", "libeinspline.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    declared = declared_functions()
+    assert len(declared) > 50
+    missing = sorted(declared - exported)
+    assert not missing, missing
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "multi_bspline.h"
+#define P(T, F) printf(#T "." #F " %zu\n", offsetof(T, F));
+#define S(T) printf(#T ".sizeof %zu\n", sizeof(T));
+int main(void) {
+  S(Ugrid) P(Ugrid, num) P(Ugrid, delta) P(Ugrid, delta_inv)
+  S(BCtype_s) S(BCtype_d) S(BCtype_c) S(BCtype_z) S(Bspline) P(Bspline, coefs)
+  S(multi_UBspline_3d_s) P(multi_UBspline_3d_s, x_grid) P(multi_UBspline_3d_s, xBC)
+  P(multi_UBspline_3d_s, num_splines)
+  S(multi_UBspline_3d_d) P(multi_UBspline_3d_d, zBC) P(multi_UBspline_3d_d, num_splines)
+  S(multi_UBspline_3d_c) P(multi_UBspline_3d_c, num_splines) P(multi_UBspline_3d_c, lapl3)
+  S(multi_UBspline_3d_z) P(multi_UBspline_3d_z, zBC) P(multi_UBspline_3d_z, lapl3)
+  return 0;
+}
+"""
+
+# x86-64 layout of the reference's definitions
+EXPECTED = {
+    "Ugrid.sizeof": 40, "Ugrid.num": 16, "Ugrid.delta": 24, "Ugrid.delta_inv": 32,
+    "BCtype_s.sizeof": 16, "BCtype_d.sizeof": 24, "BCtype_c.sizeof": 24, "BCtype_z.sizeof": 40,
+    "Bspline.sizeof": 16, "Bspline.coefs": 8,
+    "multi_UBspline_3d_s.sizeof": 216, "multi_UBspline_3d_s.x_grid": 40,
+    "multi_UBspline_3d_s.xBC": 160, "multi_UBspline_3d_s.num_splines": 208,
+    "multi_UBspline_3d_d.sizeof": 240, "multi_UBspline_3d_d.zBC": 208,
+    "multi_UBspline_3d_d.num_splines": 232,
+    "multi_UBspline_3d_c.sizeof": 248, "multi_UBspline_3d_c.num_splines": 232,
+    "multi_UBspline_3d_c.lapl3": 240,
+    "multi_UBspline_3d_z.sizeof": 296, "multi_UBspline_3d_z.zBC": 240,
+    "multi_UBspline_3d_z.lapl3": 288,
+}
+
+
+@pytest.mark.parametrize("lang", ["c", "c++"])
+def test_struct_layout_matches_reference(lang, tmp_path):
+    src = tmp_path / ("probe.c" if lang == "c" else "probe.cpp")
+    src.write_text(PROBE)
+    exe = tmp_path / "probe"
+    cc = "gcc" if lang == "c" else "g++"
+    subprocess.run([cc, "-I", INC, str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True,
+                                                         text=True, check=True).stdout.splitlines())
+    got = {k: int(v) for k, v in got.items()}
+    assert got == EXPECTED
+
+
+def test_headers_compile_standalone(tmp_path):
+    for h in sorted(os.listdir(INC)):
+        for cc, ext in (("gcc", "c"), ("g++", "cpp")):
+            f = tmp_path / f"t.{ext}"
+            f.write_text(f'#include "{h}"\nint main(void) {{ return 0; }}\n')
+            subprocess.run([cc, "-Wall", "-Werror", "-I", INC, "-fsyntax-only", str(f)], check=True)
+
+
+def test_oracle_is_not_linked_into_the_product():
+    lib = os.path.join(ROOT, "einspline_amd", "libeinspline.so")
+    out = subprocess.run(["nm", "-D", lib], capture_output=True, text=True).stdout
+    assert "oracle_" not in out
+    deps = subprocess.run(["ldd", lib], capture_output=True, text=True).stdout
+    assert "liboracle" not in deps

@@ -1,0 +1,251 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the
+same coefficient tables and points.
+
+Tolerances are normwise per point and per quantity,
+    max_n |gpu - oracle| / max_n |oracle|,
+with 1e-5 for float tables (s, c) and 1e-12 for double tables (d, z), the
+bars the task sets for matching the reference.  Tables are shared exactly:
+the oracle evaluates the very coefficients the library built, so eval parity
+is separated from create parity (tested in test_create_host.py).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import einspline_amd as ea
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"s": 1e-5, "c": 1e-5, "d": 1e-12, "z": 1e-12}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    n = ea.device_count()
+    assert n >= 1, "no HIP device visible: GPU tests must run on a GPU"
+
+
+def normwise(got, ref):
+    """Largest per-point normwise relative error."""
+    got, ref = np.asarray(got), np.asarray(ref)
+    P = ref.shape[0]
+    d = np.abs(got - ref).reshape(P, -1).max(1)
+    s = np.abs(ref).reshape(P, -1).max(1)
+    return float(np.max(d / np.maximum(s, 1e-300)))
+
+
+def make_pair(kind, M, N, rng, start=(0, 0, 0), end=(1, 1, 1), bcs=None):
+    h = ea.HostSpline(kind, M, start, end, N, bc=bcs)
+    data = rng.uniform(-1, 1, (N,) + tuple(M))
+    if kind in "cz":
+        data = data + 1j * rng.uniform(-1, 1, (N,) + tuple(M))
+    h.set_block(0, data)
+    o = oracle.Table(kind, M, start, end, N, bc=bcs)
+    o.coefs[...] = h.coefs().reshape(o.coefs.shape)
+    return h, o
+
+
+def points(rng, P, start=(0, 0, 0), end=(1, 1, 1)):
+    s, e = np.array(start, float), np.array(end, float)
+    return s + (e - s) * rng.random((P, 3)) * (1 - 1e-9)
+
+
+def gpu_eval(tab, mode, pts, packed=False):
+    P = len(pts)
+    pos = ea.DeviceBuffer.from_array(np.ascontiguousarray(pts, np.float64))
+    outs = tab.alloc_outputs(mode, P, packed)
+    tab.eval_batch(mode, pos, P, outs, packed)
+    ea.synchronize()
+    res = tab.download(mode, P, outs, packed)
+    N = tab.N
+    shaped = [res[0].reshape(P, N)]
+    if mode != "v":
+        shaped.append(res[1].reshape(P, N, 3))
+    if mode == "vgl":
+        shaped.append(res[2].reshape(P, N))
+    if mode == "vgh":
+        shaped.append(res[2].reshape(P, N, 6) if packed else res[2].reshape(P, N, 3, 3))
+    return shaped
+
+
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+@pytest.mark.parametrize("mode", ["v", "vg", "vgl", "vgh"])
+@pytest.mark.parametrize("N", [21, 64, 256])
+def test_batch_matches_oracle(kind, mode, N, rng):
+    M = (9, 8, 10)
+    start, end = (3.1, 8.7, 4.5), (9.1, 12.7, 9.3)
+    h, o = make_pair(kind, M, N, rng, start, end)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 300, start, end)
+    got = gpu_eval(tab, mode, pts)
+    ref = o.eval_batch(mode, pts)
+    for g, r in zip(got, ref):
+        assert g.shape == r.shape
+        assert normwise(g, r) < TOL[kind], (kind, mode, N)
+
+
+@pytest.mark.parametrize("kind", ["s", "d"])
+def test_nonperiodic_tables(kind, rng):
+    bcs = [(ea.NATURAL, ea.NATURAL), (ea.FLAT, ea.DERIV1, 0.0, 0.5), (ea.ANTIPERIODIC, ea.ANTIPERIODIC)]
+    M = (7, 9, 6)
+    h, o = make_pair(kind, M, 33, rng, (0, -1, 2), (1, 1, 3), bcs)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 200, (0, -1, 2), (1, 1, 3))
+    for mode in ("v", "vgl", "vgh"):
+        for g, r in zip(gpu_eval(tab, mode, pts), o.eval_batch(mode, pts)):
+            assert normwise(g, r) < TOL[kind]
+
+
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+def test_single_point_api_matches_oracle(kind, rng):
+    M, N = (6, 7, 8), 13
+    h, o = make_pair(kind, M, N, rng, (0, 0, 0), (2, 1, 3))
+    pts = points(rng, 6, (0, 0, 0), (2, 1, 3))
+    for mode in ("v", "vg", "vgl", "vgh"):
+        ref = o.eval_batch(mode, pts)
+        for p in range(len(pts)):
+            got = h.eval(mode, *pts[p])
+            for g, r in zip(got, ref):
+                assert normwise(g[None], r[p:p + 1]) < TOL[kind], (kind, mode)
+
+
+def test_single_point_sees_new_data_after_set(rng):
+    M, N = (6, 6, 6), 4
+    h, o = make_pair("d", M, N, rng)
+    x = (0.3, 0.4, 0.5)
+    v0 = h.eval("v", *x)[0]
+    h.set(1, rng.uniform(-1, 1, M))             # marks the device mirror stale
+    o.coefs[...] = h.coefs()
+    v1 = h.eval("v", *x)[0]
+    ref = o.eval_batch("v", np.array([x]))[0][0]
+    assert v1[1] != v0[1]
+    assert normwise(v1[None], ref[None]) < 1e-12
+    # direct writes to coefs need an explicit invalidate
+    h.coefs()[..., 2] *= 2.0
+    h.invalidate()
+    o.coefs[...] = h.coefs()
+    v2 = h.eval("v", *x)[0]
+    assert normwise(v2[None], o.eval_batch("v", np.array([x]))[0]) < 1e-12
+
+
+def test_packed_hessian_equals_full(rng):
+    h, _ = make_pair("s", (8, 8, 8), 64, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 128)
+    v, g, H = gpu_eval(tab, "vgh", pts)
+    vp, gp, Hp = gpu_eval(tab, "vgh", pts, packed=True)
+    np.testing.assert_array_equal(v, vp)
+    np.testing.assert_array_equal(g, gp)
+    iu = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for q, (a, b) in enumerate(iu):
+        np.testing.assert_array_equal(Hp[..., q], H[..., a, b])
+    np.testing.assert_array_equal(H, np.swapaxes(H, -1, -2))
+
+
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+def test_cuda_style_pointer_arrays(kind, rng):
+    """eval_multi_multi_UBspline_3d_X{,_vgh}_hip: device arrays of per-point
+    pointers, packed 6-entry Hessian, float positions for s/c
+    (src/multi_bspline_eval_cuda.h:62-137)."""
+    M, N, P = (7, 7, 7), 24, 50
+    h, o = make_pair(kind, M, N, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, P)
+    fpos = kind in "sc"
+    pts_used = pts.astype(np.float32).astype(np.float64) if fpos else pts
+    pos = ea.DeviceBuffer.from_array(pts.astype(np.float32) if fpos else pts)
+    nr = tab.nreal
+    es = np.dtype(tab.result_dtype).itemsize
+    vb, gb, hb = (ea.DeviceBuffer(P * k * nr * es) for k in (1, 3, 6))
+    ptrs = lambda b, k: ea.DeviceBuffer.from_array(
+        np.array([b.ptr + p * k * nr * es for p in range(P)], np.uint64))
+    vp, gp, hp = ptrs(vb, 1), ptrs(gb, 3), ptrs(hb, 6)
+    f = getattr(ea.lib, f"eval_multi_multi_UBspline_3d_{kind}_vgh_hip")
+    ea.check(f(tab.ptr, pos.ptr, vp.ptr, gp.ptr, hp.ptr, P), "vgh_hip")
+    v = vb.download(tab.result_dtype, (P, nr))
+    g = gb.download(tab.result_dtype, (P, 3 * nr))
+    hh = hb.download(tab.result_dtype, (P, 6 * nr))
+    rv, rg, rh = o.eval_batch("vgh", pts_used)
+    if kind in "cz":
+        cdt = np.complex64 if kind == "c" else np.complex128
+        v, g, hh = v.view(cdt), g.view(cdt), hh.view(cdt)
+    assert normwise(v.reshape(P, N), rv) < TOL[kind]
+    assert normwise(g.reshape(P, N, 3), rg) < TOL[kind]
+    iu = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    ref6 = np.stack([rh[..., a, b] for a, b in iu], -1)
+    assert normwise(hh.reshape(P, N, 6), ref6) < TOL[kind]
+    # value-only variant
+    f = getattr(ea.lib, f"eval_multi_multi_UBspline_3d_{kind}_hip")
+    vb.zero()
+    ea.check(f(tab.ptr, pos.ptr, vp.ptr, P), "v_hip")
+    v = vb.download(tab.result_dtype, (P, nr))
+    if kind in "cz":
+        v = v.view(cdt)
+    assert normwise(v.reshape(P, N), rv) < TOL[kind]
+
+
+@pytest.mark.parametrize("kind,lbs", [("s", (8, 16, 32)), ("d", (16, 32)), ("c", (8, 16, 32))])
+def test_tiles_are_bitwise_identical(kind, lbs, rng):
+    h, _ = make_pair(kind, (8, 9, 7), 96, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 77)
+    base = gpu_eval(tab, "vgh", pts)
+    for lb in lbs:
+        for blk in (64, 128, 256):
+            tab.set_tile(lb, blk)
+            for a, b in zip(gpu_eval(tab, "vgh", pts), base):
+                np.testing.assert_array_equal(a, b)
+    tab.set_tile(0, 0)
+
+
+def test_results_independent_of_batch(rng):
+    """A point's result is bitwise the same alone and inside any batch."""
+    h, _ = make_pair("s", (10, 10, 10), 256, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 500)
+    full = gpu_eval(tab, "vgh", pts)
+    for idx in ([0], [499], list(range(100, 137)), [3, 250, 17]):
+        part = gpu_eval(tab, "vgh", pts[idx])
+        for a, b in zip(part, full):
+            np.testing.assert_array_equal(a, b[idx])
+
+
+def test_edge_points_and_empty_batch(rng):
+    M = (6, 6, 6)
+    h, o = make_pair("d", M, 8, rng)
+    tab = ea.DeviceTable(h)
+    # grid nodes, the start corner and points just below the end
+    pts = np.array([[0, 0, 0], [0.5, 0.5, 0.5], [1 - 1e-12, 1 - 1e-12, 1 - 1e-12],
+                    [1 / 6, 2 / 6, 5 / 6]], float)
+    for g, r in zip(gpu_eval(tab, "vgh", pts), o.eval_batch("vgh", pts)):
+        assert normwise(g, r) < 1e-12
+    # x == end: the reference reads one plane past the table; here the edge
+    # cell is extended, which equals the limit from inside
+    at_end = gpu_eval(tab, "vgh", np.array([[1.0, 1.0, 1.0]]))
+    near = o.eval_batch("vgh", np.array([[1 - 1e-13] * 3]))
+    for g, r in zip(at_end, near):
+        assert normwise(g, r) < 1e-9
+    # points outside the box stay finite (memory-safe extrapolation)
+    far = gpu_eval(tab, "v", np.array([[-3.0, 0.5, 7.0]]))
+    assert np.isfinite(far[0]).all()
+    # empty batch is a no-op
+    pos = ea.DeviceBuffer(24)
+    outs = tab.alloc_outputs("v", 1)
+    tab.eval_batch("v", pos, 0, outs)
+    ea.synchronize()
+
+
+def test_errors_are_reported(rng):
+    h, _ = make_pair("s", (5, 5, 5), 4, rng)
+    tab = ea.DeviceTable(h)
+    with pytest.raises(ea.EinsplineError):
+        tab.set_tile(12, 0)
+    rc = ea.lib.einspline_eval_v_batch(tab.ptr, None, 5, None, None)
+    assert rc == ea.E_INVAL
+    # wrong-type device table for a typed entry point
+    hd, _ = make_pair("d", (5, 5, 5), 4, rng)
+    tabd = ea.DeviceTable(hd)
+    rc = ea.lib.eval_multi_multi_UBspline_3d_s_hip(tabd.ptr, None, None, 1)
+    assert rc == ea.E_INVAL
