@@ -239,6 +239,69 @@ def precision(args, res):
     res.setdefault("extra", {})["precision"] = rows
 
 
+def serving(args, res):
+    """Three resident tables (48^3 x 256 FP32, 64^3 x 512 BF16, 96^3 x 1024
+    FP32), Poisson arrivals, log-uniform 1..2048 points per request, vgh.
+    Capacity is measured closed-loop; latency at 50% and 90% of it."""
+    from einspline_amd.serve import Server
+
+    t0 = time.perf_counter()
+    t1 = build_table("s", 48, 256, seed=2024)
+    # T2/T3: random coefficient tables (contents do not change the cost)
+    rng = np.random.default_rng(77)
+    t2 = ea.HostSpline("d", (64, 64, 64), (0, 0, 0), (1, 1, 1), 512)
+    c2 = t2.coefs()
+    for i in range(c2.shape[0]):
+        c2[i] = rng.standard_normal(c2.shape[1:])
+    t3 = ea.HostSpline("s", (96, 96, 96), (0, 0, 0), (1, 1, 1), 1024)
+    c3 = t3.coefs()
+    for i in range(c3.shape[0]):
+        c3[i] = rng.standard_normal(c3.shape[1:], dtype=np.float32)
+    log(f"[bench] serving tables built in {time.perf_counter() - t0:.1f}s")
+    hosts = [(t1, ea.STORE_F32), (t2, ea.STORE_BF16), (t3, ea.STORE_F32)]
+    out = {"workload": "3 tables (48^3x256 F32, 64^3x512 BF16, 96^3x1024 F32); Poisson arrivals; "
+                       "table uniform; points log-uniform on [1,2048]; vgh; latency = submit to "
+                       "completion seen by the runtime"}
+    maxp = 2048
+    sinks = [[ea.DeviceBuffer(maxp * h.N * c * 4) for c in (1, 3, 9)] for h, _ in hosts]
+    for budget_name, budget in (("all_resident", 0), ("constrained", int(4.3e9))):
+        srv = Server(hbm_budget=budget, slots=4, max_batch_points=16384, max_batch_requests=128)
+        ids = [srv.register(h, st) for h, st in hosts]
+        # warm: admit everything that fits and build the graphs
+        srv.run_load(ids, sinks, rate_hz=0, duration_s=1.0, max_points=maxp, seed=1)
+        s_before = srv.stats()
+        lat, wall = srv.run_load(ids, sinks, rate_hz=0, duration_s=args.serve_seconds,
+                                 max_points=maxp, seed=2)
+        s_after = srv.stats()
+        cap_rps = len(lat) / wall
+        row = {"capacity_req_per_s": cap_rps,
+               "capacity_evals_per_s": (s_after["evals"] - s_before["evals"]) / wall,
+               "closed_loop_p50_ms": float(np.percentile(lat, 50) * 1e3),
+               "closed_loop_p99_ms": float(np.percentile(lat, 99) * 1e3)}
+        for frac in (0.5, 0.9):
+            before = srv.stats()
+            lat, wall = srv.run_load(ids, sinks, rate_hz=frac * cap_rps, duration_s=args.serve_seconds,
+                                     max_points=maxp, seed=int(10 * frac))
+            after = srv.stats()
+            ev = after["evals"] - before["evals"]
+            row[f"load_{int(frac * 100)}pct"] = {
+                "rate_req_per_s": frac * cap_rps, "requests": int(len(lat)),
+                "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "evals_per_s": ev / wall,
+                "evictions": after["evictions"] - before["evictions"],
+                "batches": after["batches"] - before["batches"]}
+            log(f"[bench] serving {budget_name} {int(frac * 100)}%: p50 "
+                f"{row[f'load_{int(frac * 100)}pct']['p50_ms']:.2f} ms, p99 "
+                f"{row[f'load_{int(frac * 100)}pct']['p99_ms']:.2f} ms")
+        st = srv.stats()
+        row["hbm_budget_bytes"] = budget or None
+        row["admissions"], row["evictions"] = st["admissions"], st["evictions"]
+        out[budget_name] = row
+        srv.close()
+    res.setdefault("extra", {})["serving"] = out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +309,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", action="store_true", help="add the FP32/BF16/FP16 table comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--serving", action="store_true", help="add the serving-form measurement")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
     ap.add_argument("--lane-bytes", type=int, default=0)
     ap.add_argument("--lane-bytes-16", type=int, default=0)
@@ -261,6 +326,8 @@ def main():
     res = headline(args)
     if args.precision:
         precision(args, res)
+    if args.serving:
+        serving(args, res)
     print(json.dumps(res), flush=True)
 
 

@@ -138,6 +138,8 @@ _proto("destroy_multi_UBspline_3d_hip", None, C.POINTER(multi_UBspline_3d_hip))
 _proto("destroy_Bspline", None, _vp)
 _proto("einspline_hip_invalidate", None, _vp)
 _proto("einspline_set_tile", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i)
+_proto("einspline_autotune", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i, C.POINTER(_i),
+       C.POINTER(_i), C.POINTER(C.c_float))
 _proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp)
 _proto("einspline_eval_vg_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp)
 _proto("einspline_eval_vgl_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp)
@@ -407,6 +409,13 @@ class DeviceTable:
 
     def set_tile(self, lane_bytes=0, block=0):
         check(lib.einspline_set_tile(self.ptr, int(lane_bytes), int(block)), "einspline_set_tile")
+
+    def autotune(self, mode="vgh", sample_points=8192):
+        """Pick the fastest tile for `mode` on this table (cached per shape)."""
+        lb, blk, ms = _i(), _i(), C.c_float()
+        check(lib.einspline_autotune(self.ptr, MODES.index(mode), int(sample_points), C.byref(lb),
+                                     C.byref(blk), C.byref(ms)), "einspline_autotune")
+        return lb.value, blk.value, ms.value
 
     def out_sizes(self, mode, packed=False):
         """Reals per point of (vals, grads, lapl/hess)."""

@@ -263,19 +263,14 @@ struct Occupancy {
   static constexpr int waves = regs <= 116 ? ESK_WAVES_PER_EU : regs <= 160 ? 3 : regs <= 248 ? 2 : 1;
 };
 
+// Evaluate chunk `ch` of point `p` (0 <= p < P.npoints, 0 <= ch < P.chunks).
+// With UNIF, p must be the same for the whole wavefront.
 template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS>
-__global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) eval3d(const Params P) {
+__device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   using A = typename acc_of<S>::type;
   using L = Loader<S, LB>;
   constexpr int V = L::V;
   static_assert(!CPLX || (V % 2 == 0), "complex lanes hold whole orbitals");
-
-  // npoints * chunks < 2^31 is checked on the host, so 32-bit math suffices
-  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
-  int p = (int)(gid / (unsigned)P.chunks);
-  if (UNIF) p = uniform(p);
-  const int ch = (int)(gid - (unsigned)p * (unsigned)P.chunks);
-  if (p >= P.npoints) return;
 
   const PosT *pp = (const PosT *)P.pos + 3l * p;
   const double x = (double)pp[0], y = (double)pp[1], z = (double)pp[2];
@@ -502,6 +497,17 @@ __global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) ev
       else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
     }
   }
+}
+
+template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS>
+__global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) eval3d(const Params P) {
+  // npoints * chunks < 2^31 is checked on the host, so 32-bit math suffices
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = (int)(gid / (unsigned)P.chunks);
+  if (UNIF) p = uniform(p);
+  const int ch = (int)(gid - (unsigned)p * (unsigned)P.chunks);
+  if (p >= P.npoints) return;
+  eval_chunk<S, LB, MODE, CPLX, UNIF, PosT, PTRS>(P, p, ch);
 }
 
 }  // namespace esk

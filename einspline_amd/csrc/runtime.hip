@@ -11,11 +11,15 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "multi_bspline.h"
 #include "einspline_private.h"
 #include "launch.hpp"
+#include "runtime_internal.hpp"
 
 using esk::Launch;
 using esk::Params;
@@ -25,7 +29,7 @@ using esk::Params;
 
 static thread_local char g_err[512] = "no error";
 
-static int fail(int code, const char *fmt, ...) {
+int esk_fail(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);
@@ -33,9 +37,12 @@ static int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-static int hip_fail(hipError_t e, const char *what) {
+#define fail esk_fail
+
+int esk_hip_fail(hipError_t e, const char *what) {
   return fail(EINSPLINE_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
+#define hip_fail esk_hip_fail
 
 extern "C" const char *einspline_hip_last_error(void) { return g_err; }
 
@@ -259,6 +266,141 @@ extern "C" int einspline_set_tile(multi_UBspline_3d_hip *t, int lane_bytes, int 
 }
 
 // ---------------------------------------------------------------------------
+// autotuner
+
+namespace {
+std::mutex g_tune_mu;
+struct TuneKey {
+  int storage, tcode, mode, row_bytes;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(storage, tcode, mode, row_bytes) < std::tie(o.storage, o.tcode, o.mode, o.row_bytes);
+  }
+};
+std::map<TuneKey, std::pair<int, int>> g_tune;
+bool g_tune_loaded = false;
+
+void tune_load() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char *f = getenv("EINSPLINE_TUNE_CACHE");
+  if (!f) return;
+  FILE *fp = fopen(f, "r");
+  if (!fp) return;
+  TuneKey k;
+  int lb, blk;
+  while (fscanf(fp, "%d %d %d %d %d %d", &k.storage, &k.tcode, &k.mode, &k.row_bytes, &lb, &blk) == 6)
+    g_tune[k] = {lb, blk};
+  fclose(fp);
+}
+
+void tune_save() {
+  const char *f = getenv("EINSPLINE_TUNE_CACHE");
+  if (!f) return;
+  FILE *fp = fopen(f, "w");
+  if (!fp) return;
+  for (auto &kv : g_tune)
+    fprintf(fp, "%d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
+            kv.first.row_bytes, kv.second.first, kv.second.second);
+  fclose(fp);
+}
+}  // namespace
+
+static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
+                       void *v, void *g, void *h, int packed, void *stream);
+
+extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample, int *lb_out,
+                                  int *blk_out, float *ms_out) {
+  if (!t || mode < 0 || mode > 3) return fail(EINSPLINE_EINVAL, "bad autotune arguments");
+  if (sample <= 0) sample = 8192;
+  const int eb = (int)storage_bytes(t->storage);
+  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_load();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      einspline_set_tile(t, it->second.first, it->second.second);
+      if (lb_out) *lb_out = it->second.first;
+      if (blk_out) *blk_out = it->second.second;
+      if (ms_out) *ms_out = -1.0f;
+      return EINSPLINE_OK;
+    }
+  }
+  // sample points: deterministic, uniform in the box
+  std::vector<double> pos(3 * (size_t)sample);
+  uint64_t x = 0x9e3779b97f4a7c15ull;
+  const Ugrid *gr[3] = {&t->x_grid, &t->y_grid, &t->z_grid};
+  for (size_t i = 0; i < pos.size(); ++i) {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    const Ugrid *g = gr[i % 3];
+    pos[i] = g->start + u * (g->end - g->start) * (1.0 - 1e-12);
+  }
+  const size_t nreal = (size_t)result_reals(t);
+  const size_t es = t->storage == EINSPLINE_STORE_F64 ? 8 : 4;
+  void *dp = nullptr, *dv = nullptr, *dg = nullptr, *dh = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st = nullptr;
+  int rc = EINSPLINE_OK;
+  hipError_t e;
+  if ((e = hipMalloc(&dp, pos.size() * 8)) != hipSuccess ||
+      (e = hipMalloc(&dv, sample * nreal * es)) != hipSuccess ||
+      (e = hipMalloc(&dg, 3 * sample * nreal * es)) != hipSuccess ||
+      (e = hipMalloc(&dh, 9 * sample * nreal * es)) != hipSuccess ||
+      (e = hipMemcpy(dp, pos.data(), pos.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+    rc = hip_fail(e, "autotune setup");
+  TableExt *ext = (TableExt *)t;
+  const int old_lb = ext->lane_bytes, old_blk = ext->block;
+  int best_lb = 16, best_blk = 256;
+  float best = -1.0f;
+  const int lbs_f64[] = {16, 32}, lbs_other[] = {8, 16, 32};
+  const int *lbs = t->storage == EINSPLINE_STORE_F64 ? lbs_f64 : lbs_other;
+  const int nlb = t->storage == EINSPLINE_STORE_F64 ? 2 : 3;
+  const int blks[] = {64, 128, 256};
+  for (int a = 0; rc == EINSPLINE_OK && a < nlb; ++a) {
+    if ((t->row_reals * eb) % lbs[a]) continue;
+    for (int b = 0; rc == EINSPLINE_OK && b < 3; ++b) {
+      ext->lane_bytes = lbs[a];
+      ext->block = blks[b];
+      auto launch = [&]() {
+        return run_strided(t, mode, (const double *)dp, sample, dv, dg, dh, 0, st);
+      };
+      if ((rc = launch()) != EINSPLINE_OK) break;       // warm-up
+      hipEventRecord(e0, st);
+      for (int r = 0; r < 5 && rc == EINSPLINE_OK; ++r) rc = launch();
+      hipEventRecord(e1, st);
+      if ((e = hipEventSynchronize(e1)) != hipSuccess) { rc = hip_fail(e, "autotune"); break; }
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      ms /= 5.0f;
+      if (best < 0.0f || ms < best) { best = ms; best_lb = lbs[a]; best_blk = blks[b]; }
+    }
+  }
+  ext->lane_bytes = old_lb;
+  ext->block = old_blk;
+  if (dp) hipFree(dp);
+  if (dv) hipFree(dv);
+  if (dg) hipFree(dg);
+  if (dh) hipFree(dh);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (rc != EINSPLINE_OK) return rc;
+  einspline_set_tile(t, best_lb, best_blk);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = {best_lb, best_blk};
+    tune_save();
+  }
+  if (lb_out) *lb_out = best_lb;
+  if (blk_out) *blk_out = best_blk;
+  if (ms_out) *ms_out = best;
+  return EINSPLINE_OK;
+}
+
+// ---------------------------------------------------------------------------
 // launching
 
 static int result_reals(const multi_UBspline_3d_hip *t) {
@@ -266,7 +408,7 @@ static int result_reals(const multi_UBspline_3d_hip *t) {
 }
 
 // Fill the table-dependent kernel arguments and the tile.
-static int prepare(const multi_UBspline_3d_hip *t, int num, Launch &L, Params &P) {
+int esk_prepare(const multi_UBspline_3d_hip *t, int num, Launch &L, Params &P) {
   if (!t || !t->coefs) return fail(EINSPLINE_EINVAL, "NULL device table");
   if (num < 0) return fail(EINSPLINE_EINVAL, "negative point count");
   const TableExt *x = (const TableExt *)t;
@@ -309,7 +451,7 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
                        void *v, void *g, void *h, int packed, void *stream) {
   Launch L;
   Params P;
-  int rc = prepare(t, num, L, P);
+  int rc = esk_prepare(t, num, L, P);
   if (rc) return rc;
   if (num == 0) return EINSPLINE_OK;
   if (!pos || !v || (mode >= esk::MODE_VG && !g) || (mode >= esk::MODE_VGL && !h))
@@ -353,7 +495,7 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
     return fail(EINSPLINE_EINVAL, "F64 storage in a single-precision table");
   Launch L;
   Params P;
-  int rc = prepare(t, num, L, P);
+  int rc = esk_prepare(t, num, L, P);
   if (rc) return rc;
   if (num == 0) return EINSPLINE_OK;
   if (!pos || !v || (mode == esk::MODE_VGH && (!g || !h)))

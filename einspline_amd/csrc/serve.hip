@@ -1,0 +1,613 @@
+// serve.hip -- serving runtime (include/einspline_serve.h).
+//
+// One worker thread owns the GPU side.  It keeps `slots` batches in flight,
+// each on its own stream with its own pinned staging buffer, device buffer
+// and instantiated hipGraph.  A batch is a list of segments (one per
+// request) grouped by kernel variant; the graph holds one persistent kernel
+// per non-empty group.  The persistent kernels read the batch descriptor
+// from device memory, so one captured graph serves every batch with the same
+// set of groups, whatever its size.  Each wavefront walks a strided range
+// of "wave work units" (64 lanes of one segment) until the group is done,
+// so every wavefront reaches the exit.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "einspline_serve.h"
+#include "einspline_private.h"
+#include "eval_kernel.hpp"
+#include "runtime_internal.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+// kernel groups: storage (F32, BF16, F16) x (one point per wavefront or not)
+constexpr int kGroups = 6;
+constexpr int kMaxSeg = 256;
+
+struct Seg {
+  esk::Params P;   // table, positions and outputs of one request
+  int wave0;       // first wave work unit of the segment within its group
+  int waves;
+};
+
+struct BatchDesc {
+  int nseg[kGroups];
+  int seg0[kGroups];
+  int total_waves[kGroups];
+  int pad[2];
+  Seg seg[kMaxSeg];
+};
+
+template <class S, bool UNIF>
+__global__ void __launch_bounds__(256, (esk::Occupancy<S, 16, esk::MODE_VGH, UNIF>::waves))
+serve_vgh(const BatchDesc *__restrict__ B, int g) {
+  const int lane = threadIdx.x & 63;
+  const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  const int total = B->total_waves[g];
+  const int s0 = B->seg0[g], ns = B->nseg[g];
+  int s = 0;
+  for (int w = esk::uniform(wv); w < total; w += nw) {
+    while (s + 1 < ns && B->seg[s0 + s + 1].wave0 <= w) ++s;
+    s = esk::uniform(s);
+    const Seg &sg = B->seg[s0 + s];
+    const int chunks = sg.P.chunks;
+    const int lid = (w - sg.wave0) * 64 + lane;
+    int p = lid / chunks;
+    if (UNIF) p = esk::uniform(p);
+    const int ch = lid - p * chunks;
+    if (p < sg.P.npoints)
+      esk::eval_chunk<S, 16, esk::MODE_VGH, false, UNIF, double, false>(sg.P, p, ch);
+  }
+}
+
+typedef void (*serve_fn)(const BatchDesc *, int);
+
+serve_fn group_kernel(int g) {
+  switch (g) {
+    case 0: return serve_vgh<float, true>;
+    case 1: return serve_vgh<float, false>;
+    case 2: return serve_vgh<esk::bf16s, true>;
+    case 3: return serve_vgh<esk::bf16s, false>;
+    case 4: return serve_vgh<esk::f16s, true>;
+    default: return serve_vgh<esk::f16s, false>;
+  }
+}
+
+int storage_group(einspline_storage st) {
+  return st == EINSPLINE_STORE_F32 ? 0 : st == EINSPLINE_STORE_BF16 ? 2 : 4;
+}
+
+struct Table {
+  void *host = nullptr;
+  bool from_double = false;
+  einspline_storage storage = EINSPLINE_STORE_F32;
+  size_t bytes = 0;                 // device bytes once admitted
+  multi_UBspline_3d_hip *dev = nullptr;
+  uint64_t last_use = 0;
+  int inflight = 0;                 // requests of this table on the GPU
+  int nsplines = 0;
+};
+
+struct Req {
+  uint64_t id = 0;
+  int table = 0;
+  int npts = 0;
+  std::vector<double> pos;
+  void *v = nullptr, *g = nullptr, *h = nullptr;
+  clk::time_point t0;
+  double latency = -1.0;
+  bool done = false;
+  bool collect = true;              // false: nobody waits, drop at completion
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  unsigned char *hbuf = nullptr;    // pinned: descriptor + positions
+  unsigned char *dbuf = nullptr;
+  size_t bytes = 0;
+  std::map<unsigned, hipGraphExec_t> graphs;   // by non-empty group mask
+  bool busy = false;
+  std::vector<Req *> reqs;
+};
+
+}  // namespace
+
+struct einspline_server {
+  einspline_server_config cfg{};
+  int device = 0;
+  int blocks[kGroups] = {};         // persistent grid per group
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req *> pending;
+  std::unordered_map<uint64_t, Req *> live;
+  std::vector<Table> tables;
+  std::vector<Slot> slots;
+  std::thread worker;
+  bool stop = false;
+  int worker_error = 0;
+  uint64_t next_id = 1, use_clock = 0;
+  size_t budget = 0, used = 0;
+  einspline_server_stats st{};
+  std::vector<double> lat_log;      // latencies of completed requests (seconds)
+
+  int init(const einspline_server_config *c);
+  void run();
+  void poll_completions();
+  bool launch_batch(Slot &s, std::unique_lock<std::mutex> &lk);
+  int admit(int tid, std::unique_lock<std::mutex> &lk, const std::vector<char> &pinned);
+  hipGraphExec_t graph_for(Slot &s, unsigned mask);
+};
+
+// ---------------------------------------------------------------------------
+
+static size_t predicted_bytes(const Table &t) {
+  // same padding rule as the device upload: rows padded to 32 bytes
+  const int eb = t.storage == EINSPLINE_STORE_F32 ? 4 : 2;
+  int Nx, Ny, Nz, N;
+  if (t.from_double) {
+    const multi_UBspline_3d_d *s = (const multi_UBspline_3d_d *)t.host;
+    Nz = (int)(s->y_stride / s->z_stride); Ny = (int)(s->x_stride / s->y_stride);
+    N = s->num_splines;
+  } else {
+    const multi_UBspline_3d_s *s = (const multi_UBspline_3d_s *)t.host;
+    Nz = (int)(s->y_stride / s->z_stride); Ny = (int)(s->x_stride / s->y_stride);
+    N = s->num_splines;
+  }
+  const einspline_host3d *h = einspline_host_of(t.host);
+  Nx = h ? h->Nx : 0;
+  const size_t per32 = 32 / eb;
+  const size_t row = ((size_t)N + per32 - 1) / per32 * per32;
+  return (size_t)Nx * Ny * Nz * row * eb;
+}
+
+int einspline_server::init(const einspline_server_config *c) {
+  cfg = *c;
+  if (cfg.slots <= 0) cfg.slots = 4;
+  if (cfg.max_batch_points <= 0) cfg.max_batch_points = 16384;
+  if (cfg.max_batch_requests <= 0) cfg.max_batch_requests = 256;
+  cfg.max_batch_requests = std::min(cfg.max_batch_requests, kMaxSeg);
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return esk_hip_fail(e, "hipGetDevice");
+  if (cfg.hbm_budget == 0) {
+    size_t fr = 0, tot = 0;
+    e = hipMemGetInfo(&fr, &tot);
+    if (e != hipSuccess) return esk_hip_fail(e, "hipMemGetInfo");
+    cfg.hbm_budget = fr / 10 * 9;
+  }
+  budget = cfg.hbm_budget;
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return esk_hip_fail(e, "hipGetDeviceProperties");
+  for (int g = 0; g < kGroups; ++g) {
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)group_kernel(g), 256, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    blocks[g] = prop.multiProcessorCount * per_cu;
+  }
+  const size_t bytes = sizeof(BatchDesc) + (size_t)cfg.max_batch_points * 3 * sizeof(double);
+  slots.resize(cfg.slots);
+  for (Slot &s : slots) {
+    s.bytes = bytes;
+    if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&s.hbuf, bytes, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc((void **)&s.dbuf, bytes)) != hipSuccess)
+      return esk_hip_fail(e, "server slot setup");
+  }
+  worker = std::thread([this] { run(); });
+  return EINSPLINE_OK;
+}
+
+hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
+  auto it = s.graphs.find(mask);
+  if (it != s.graphs.end()) return it->second;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  const BatchDesc *d = (const BatchDesc *)s.dbuf;
+  for (int g = 0; g < kGroups; ++g)
+    if (mask & (1u << g))
+      hipLaunchKernelGGL(group_kernel(g), dim3(blocks[g]), dim3(256), 0, s.stream, d, g);
+  if (hipStreamEndCapture(s.stream, &graph) != hipSuccess) return nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return nullptr;
+  s.graphs[mask] = exec;
+  return exec;
+}
+
+// Make table `tid` resident.  Returns 1 admitted/resident, 0 deferred.
+int einspline_server::admit(int tid, std::unique_lock<std::mutex> &lk,
+                            const std::vector<char> &pinned) {
+  Table &t = tables[tid];
+  if (t.dev) return 1;
+  const size_t need = predicted_bytes(t);
+  if (need > budget) return -1;
+  while (used + need > budget) {
+    // evict the least recently used table with nothing in flight
+    int victim = -1;
+    for (int i = 0; i < (int)tables.size(); ++i)
+      if (tables[i].dev && tables[i].inflight == 0 && !pinned[i] &&
+          (victim < 0 || tables[i].last_use < tables[victim].last_use))
+        victim = i;
+    if (victim < 0) return 0;
+    destroy_multi_UBspline_3d_hip(tables[victim].dev);
+    tables[victim].dev = nullptr;
+    used -= tables[victim].bytes;
+    st.evictions++;
+  }
+  used += need;                         // reserve before dropping the lock
+  lk.unlock();
+  multi_UBspline_3d_hip *d =
+      t.from_double ? create_multi_UBspline_3d_s_hip_store((const multi_UBspline_3d_d *)t.host, t.storage)
+                    : create_multi_UBspline_3d_s_hip((multi_UBspline_3d_s *)t.host);
+  lk.lock();
+  if (!d) { used -= need; return 0; }
+  t.dev = d;
+  t.bytes = d->bytes;
+  used = used - need + d->bytes;
+  st.admissions++;
+  return 1;
+}
+
+void einspline_server::poll_completions() {
+  for (Slot &s : slots) {
+    if (!s.busy) continue;
+    hipError_t e = hipEventQuery(s.done);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) worker_error = esk_hip_fail(e, "server batch");
+    const auto now = clk::now();
+    for (Req *r : s.reqs) {
+      r->latency = std::chrono::duration<double>(now - r->t0).count();
+      r->done = true;
+      tables[r->table].inflight--;
+      st.completed++;
+      st.evals += (double)r->npts * tables[r->table].nsplines;
+      lat_log.push_back(r->latency);
+      if (!r->collect) { live.erase(r->id); delete r; }
+    }
+    s.reqs.clear();
+    s.busy = false;
+  }
+}
+
+// Form and launch one batch on slot s from the pending queue.
+bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
+  std::vector<char> pinned(tables.size(), 0);
+  std::vector<Req *> take;
+  int points = 0;
+  // Only this thread removes from `pending`; submitters append.  Work on a
+  // snapshot of the head so admit() may drop the lock safely.
+  const size_t look = std::min(pending.size(), (size_t)cfg.max_batch_requests);
+  std::vector<Req *> head(pending.begin(), pending.begin() + look);
+  for (Req *r : head) {
+    if (points + r->npts > cfg.max_batch_points || (int)take.size() >= cfg.max_batch_requests)
+      continue;
+    pinned[r->table] = 1;
+    const int a = admit(r->table, lk, pinned);
+    if (a <= 0) {
+      pinned[r->table] = 0;
+      st.deferrals++;
+      continue;
+    }
+    take.push_back(r);
+    points += r->npts;
+  }
+  if (!take.empty()) {
+    auto is_taken = [&](Req *r) { return std::find(take.begin(), take.end(), r) != take.end(); };
+    pending.erase(std::remove_if(pending.begin(), pending.begin() + look, is_taken),
+                  pending.begin() + look);
+  }
+  if (take.empty()) return false;
+  // group segments by kernel variant
+  BatchDesc *B = (BatchDesc *)s.hbuf;
+  double *hpos = (double *)(s.hbuf + sizeof(BatchDesc));
+  const double *dpos = (const double *)(s.dbuf + sizeof(BatchDesc));
+  std::vector<std::pair<int, Req *>> order;
+  std::vector<esk::Params> params(take.size());
+  for (size_t i = 0; i < take.size(); ++i) {
+    Req *r = take[i];
+    Table &t = tables[r->table];
+    esk::Launch L;
+    esk_prepare(t.dev, r->npts, L, params[i]);
+    // serving kernels use 16-byte lanes
+    params[i].chunks = t.dev->row_reals * (t.storage == EINSPLINE_STORE_F32 ? 4 : 2) / 16;
+    const int g = storage_group(t.storage) + ((params[i].chunks % 64) ? 1 : 0);
+    order.push_back({g, r});
+    params[i].npoints = r->npts;
+  }
+  std::vector<size_t> idx(take.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return order[a].first < order[b].first; });
+  memset(B, 0, offsetof(BatchDesc, seg));
+  int off = 0;
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const size_t i = idx[k];
+    Req *r = order[i].second;
+    const int g = order[i].first;
+    if (B->nseg[g] == 0) B->seg0[g] = (int)k;
+    Seg &sg = B->seg[k];
+    sg.P = params[i];
+    memcpy(hpos + 3 * (size_t)off, r->pos.data(), sizeof(double) * 3 * r->npts);
+    sg.P.pos = dpos + 3 * (size_t)off;
+    sg.P.out_v = r->v; sg.P.out_g = r->g; sg.P.out_h = r->h;
+    sg.P.ld_v = sg.P.nreal;
+    sg.P.ld_g = 3l * sg.P.nreal;
+    sg.P.ld_h = (cfg.hess_packed ? 6l : 9l) * sg.P.nreal;
+    sg.P.hpacked = cfg.hess_packed ? 1 : 0;
+    sg.wave0 = B->total_waves[g];
+    sg.waves = (int)(((long)r->npts * sg.P.chunks + 63) / 64);
+    B->total_waves[g] += sg.waves;
+    B->nseg[g]++;
+    off += r->npts;
+    tables[r->table].inflight++;
+    tables[r->table].last_use = ++use_clock;
+  }
+  unsigned mask = 0;
+  for (int g = 0; g < kGroups; ++g)
+    if (B->nseg[g]) mask |= 1u << g;
+  hipGraphExec_t exec = graph_for(s, mask);
+  const size_t bytes = sizeof(BatchDesc) + sizeof(double) * 3 * (size_t)off;
+  hipError_t e = exec ? hipMemcpyAsync(s.dbuf, s.hbuf, bytes, hipMemcpyHostToDevice, s.stream)
+                      : hipErrorInvalidValue;
+  if (e == hipSuccess) e = hipGraphLaunch(exec, s.stream);
+  if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+  if (e != hipSuccess) worker_error = esk_hip_fail(e, "server launch");
+  s.reqs = std::move(take);
+  s.busy = true;
+  st.batches++;
+  return true;
+}
+
+void einspline_server::run() {
+  hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    poll_completions();
+    cv_done.notify_all();
+    bool any_busy = false;
+    for (Slot &s : slots) any_busy |= s.busy;
+    if (stop && pending.empty() && !any_busy) break;
+    Slot *free_slot = nullptr;
+    for (Slot &s : slots)
+      if (!s.busy) { free_slot = &s; break; }
+    bool launched = false;
+    if (free_slot && !pending.empty()) launched = launch_batch(*free_slot, lk);
+    if (!launched) {
+      if (any_busy || (free_slot && !pending.empty())) {
+        // completions are polled: yield briefly without sleeping long
+        lk.unlock();
+        std::this_thread::yield();
+        lk.lock();
+      } else {
+        cv_work.wait_for(lk, std::chrono::milliseconds(5));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C API
+
+extern "C" einspline_server *einspline_server_create(const einspline_server_config *cfg) {
+  einspline_server_config def{};
+  einspline_server *s = new einspline_server();
+  if (s->init(cfg ? cfg : &def) != EINSPLINE_OK) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+extern "C" void einspline_server_destroy(einspline_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (Slot &sl : s->slots) {
+    for (auto &kv : sl.graphs) hipGraphExecDestroy(kv.second);
+    if (sl.stream) hipStreamDestroy(sl.stream);
+    if (sl.done) hipEventDestroy(sl.done);
+    if (sl.hbuf) hipHostFree(sl.hbuf);
+    if (sl.dbuf) hipFree(sl.dbuf);
+  }
+  for (Table &t : s->tables)
+    if (t.dev) destroy_multi_UBspline_3d_hip(t.dev);
+  for (auto &kv : s->live) delete kv.second;
+  delete s;
+}
+
+extern "C" int einspline_server_register_table(einspline_server *s, void *host,
+                                               einspline_storage storage, int *table_id) {
+  if (!s || !host || !table_id) return esk_fail(EINSPLINE_EINVAL, "NULL argument");
+  einspline_host3d *h = einspline_host_of(host);
+  if (!h) return esk_fail(EINSPLINE_EINVAL, "spline was not created by this library");
+  Table t;
+  t.host = host;
+  t.storage = storage;
+  const type_code tc = h->pub.s.tcode;
+  if (tc == SINGLE_REAL && storage == EINSPLINE_STORE_F32) t.from_double = false;
+  else if (tc == DOUBLE_REAL && storage != EINSPLINE_STORE_F64) t.from_double = true;
+  else return esk_fail(EINSPLINE_EINVAL, "serving supports real float tables (F32) or double tables stored as F32/BF16/F16");
+  t.nsplines = h->nsplines;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (predicted_bytes(t) > s->budget)
+    return esk_fail(EINSPLINE_EBUDGET, "table of %zu bytes exceeds the HBM budget of %zu bytes",
+                    predicted_bytes(t), s->budget);
+  s->tables.push_back(t);
+  *table_id = (int)s->tables.size() - 1;
+  return EINSPLINE_OK;
+}
+
+static int submit_impl(einspline_server *s, int tid, const double *pos, int npts, void *v, void *g,
+                       void *h, uint64_t *ticket) {
+  if (!s || !pos || !v || !g || !h) return esk_fail(EINSPLINE_EINVAL, "NULL argument");
+  if (npts < 1 || npts > s->cfg.max_batch_points)
+    return esk_fail(EINSPLINE_EINVAL, "npts must be in 1..%d", s->cfg.max_batch_points);
+  Req *r = new Req();
+  r->npts = npts;
+  r->pos.assign(pos, pos + 3 * (size_t)npts);
+  r->v = v; r->g = g; r->h = h;
+  r->collect = ticket != nullptr;
+  {
+    std::lock_guard<std::mutex> lg(s->mu);
+    if (tid < 0 || tid >= (int)s->tables.size()) {
+      delete r;
+      return esk_fail(EINSPLINE_EINVAL, "unknown table id %d", tid);
+    }
+    if (s->worker_error) {
+      delete r;
+      return s->worker_error;
+    }
+    r->table = tid;
+    r->id = s->next_id++;
+    r->t0 = clk::now();
+    s->live[r->id] = r;
+    s->pending.push_back(r);
+    s->st.submitted++;
+    if (ticket) *ticket = r->id;
+  }
+  s->cv_work.notify_one();
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_server_submit(einspline_server *s, int tid, const double *pos, int npts,
+                                       void *v, void *g, void *h, uint64_t *ticket) {
+  return submit_impl(s, tid, pos, npts, v, g, h, ticket);
+}
+
+static int finish(einspline_server *s, uint64_t id, double *lat, bool block) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->live.find(id);
+  if (it == s->live.end()) return esk_fail(EINSPLINE_EINVAL, "unknown or already collected ticket");
+  Req *r = it->second;
+  if (!r->done) {
+    if (!block) return 0;
+    s->cv_done.wait(lk, [&] { return r->done || s->worker_error; });
+    if (!r->done) return s->worker_error;
+  }
+  if (lat) *lat = r->latency;
+  s->live.erase(it);
+  delete r;
+  return 1;
+}
+
+extern "C" int einspline_server_test(einspline_server *s, uint64_t t, double *lat) {
+  if (!s) return esk_fail(EINSPLINE_EINVAL, "NULL server");
+  return finish(s, t, lat, false);
+}
+
+extern "C" int einspline_server_wait(einspline_server *s, uint64_t t, double *lat) {
+  if (!s) return esk_fail(EINSPLINE_EINVAL, "NULL server");
+  int rc = finish(s, t, lat, true);
+  return rc == 1 ? EINSPLINE_OK : rc;
+}
+
+extern "C" int einspline_server_drain(einspline_server *s) {
+  if (!s) return esk_fail(EINSPLINE_EINVAL, "NULL server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] {
+    if (s->worker_error) return true;
+    if (!s->pending.empty()) return false;
+    for (Slot &sl : s->slots)
+      if (sl.busy) return false;
+    return true;
+  });
+  return s->worker_error;
+}
+
+extern "C" int einspline_server_get_stats(einspline_server *s, einspline_server_stats *out) {
+  if (!s || !out) return esk_fail(EINSPLINE_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  *out = s->st;
+  out->resident_tables = 0;
+  for (Table &t : s->tables) out->resident_tables += t.dev != nullptr;
+  out->resident_bytes = s->used;
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_server_table_resident(einspline_server *s, int tid) {
+  if (!s) return esk_fail(EINSPLINE_EINVAL, "NULL server");
+  std::lock_guard<std::mutex> g(s->mu);
+  if (tid < 0 || tid >= (int)s->tables.size()) return esk_fail(EINSPLINE_EINVAL, "unknown table");
+  return s->tables[tid].dev != nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// native load generator (benchmark helper)
+
+extern "C" int einspline_server_run_load(einspline_server *s, const einspline_load *ld,
+                                         double *latencies, size_t cap, size_t *nreq,
+                                         double *wall_s) {
+  if (!s || !ld || ld->ntables < 1 || !ld->table_ids) return esk_fail(EINSPLINE_EINVAL, "bad load");
+  std::mt19937_64 rng(ld->seed);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::exponential_distribution<double> gap(ld->rate_hz > 0 ? ld->rate_hz : 1.0);
+  const int maxp = ld->max_points > 0 ? ld->max_points : 2048;
+  std::vector<double> pos(3 * (size_t)maxp);
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->lat_log.clear();
+  }
+  const auto t_start = clk::now();
+  double t_next = 0.0;
+  size_t n = 0;
+  for (;;) {
+    if (ld->rate_hz > 0) t_next += gap(rng);
+    if (t_next >= ld->duration_s) break;
+    if (ld->rate_hz > 0) {
+      while (std::chrono::duration<double>(clk::now() - t_start).count() < t_next) {
+      }
+    } else if (std::chrono::duration<double>(clk::now() - t_start).count() >= ld->duration_s) {
+      break;
+    }
+    // closed loop at saturation: bound the queue
+    if (ld->rate_hz <= 0) {
+      for (;;) {
+        size_t q;
+        {
+          std::lock_guard<std::mutex> g(s->mu);
+          q = s->pending.size();
+        }
+        if (q < (size_t)s->cfg.max_batch_requests * s->slots.size()) break;
+        std::this_thread::yield();
+      }
+    }
+    const int k = (int)(U(rng) * ld->ntables) % ld->ntables;
+    // log-uniform point count on [1, maxp]
+    const int np = std::min(maxp, std::max(1, (int)std::exp(U(rng) * std::log((double)maxp + 1.0))));
+    for (int i = 0; i < 3 * np; ++i) pos[i] = U(rng) * ld->box[i % 3];
+    int rc = submit_impl(s, ld->table_ids[k], pos.data(), np, ld->vals_d[k], ld->grads_d[k],
+                         ld->hess_d[k], nullptr);
+    if (rc) return rc;
+    ++n;
+  }
+  int rc = einspline_server_drain(s);
+  const double wall = std::chrono::duration<double>(clk::now() - t_start).count();
+  std::lock_guard<std::mutex> g(s->mu);
+  const size_t m = std::min(cap, s->lat_log.size());
+  if (latencies) std::copy(s->lat_log.begin(), s->lat_log.begin() + m, latencies);
+  if (nreq) *nreq = s->lat_log.size();
+  if (wall_s) *wall_s = wall;
+  return rc;
+}

@@ -148,6 +148,15 @@ int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *spline, const double *
  * launches on `spline`; returns EINSPLINE_EINVAL for unsupported shapes. */
 int einspline_set_tile(multi_UBspline_3d_hip *spline, int lane_bytes, int block_threads);
 
+/* Autotuner (no reference counterpart): times every supported tile shape
+ * (lane_bytes x block_threads) of `mode` (0 v, 1 vg, 2 vgl, 3 vgh) on
+ * `sample_points` uniform random points of the table's box, applies the
+ * fastest with einspline_set_tile and returns it.  Results are cached per
+ * (storage, type, mode, row bytes) for the process, and in the text file
+ * named by $EINSPLINE_TUNE_CACHE when set.  Synchronises the device. */
+int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_points,
+                       int *lane_bytes, int *block_threads, float *best_ms);
+
 /* ---- single-point path support ----------------------------------------
  * The single-point eval_multi_UBspline_3d_* functions keep a device mirror
  * of the host table and refresh it after set_*.  A caller that writes

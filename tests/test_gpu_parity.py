@@ -249,3 +249,19 @@ def test_errors_are_reported(rng):
     tabd = ea.DeviceTable(hd)
     rc = ea.lib.eval_multi_multi_UBspline_3d_s_hip(tabd.ptr, None, None, 1)
     assert rc == ea.E_INVAL
+
+
+def test_autotune_picks_a_valid_tile_and_keeps_results(rng):
+    h, o = make_pair("s", (10, 9, 8), 256, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 200)
+    before = gpu_eval(tab, "vgh", pts)
+    lb, blk, ms = tab.autotune("vgh", 4096)
+    assert lb in (8, 16, 32) and blk in (64, 128, 256) and ms > 0
+    after = gpu_eval(tab, "vgh", pts)
+    for a, b in zip(after, before):
+        np.testing.assert_array_equal(a, b)
+    # second call on the same shape hits the cache
+    tab2 = ea.DeviceTable(h)
+    lb2, blk2, ms2 = tab2.autotune("vgh", 4096)
+    assert (lb2, blk2) == (lb, blk) and ms2 < 0
