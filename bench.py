@@ -182,7 +182,7 @@ def headline(args):
         log("[bench] CPU baseline (oracle, host cores)")
         res["cpu_baseline"] = cpu_baseline("s", host, pts, "vgh")
     # layout variant for the record
-    if not packed:
+    if not packed and not args.no_extra:
         outs_p = tab.alloc_outputs("vgh", P, True)
         w2, k2 = time_batches(tab, "vgh", pos, P, outs_p, max(5, args.steps // 2), 2, True)
         res.setdefault("extra", {})["f32_vgh_packed6"] = {
@@ -310,6 +310,7 @@ def main():
     ap.add_argument("--precision", action="store_true", help="add the FP32/BF16/FP16 table comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--serving", action="store_true", help="add the serving-form measurement")
+    ap.add_argument("--no-extra", action="store_true", help="skip the packed-Hessian side measurement")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
     ap.add_argument("--lane-bytes", type=int, default=0)

@@ -160,6 +160,7 @@ _proto("einspline_hip_event_destroy", _i, _vp)
 _proto("einspline_hip_event_record", _i, _vp, _vp)
 _proto("einspline_hip_event_synchronize", _i, _vp)
 _proto("einspline_hip_event_elapsed_ms", C.c_float, _vp, _vp)
+_proto("einspline_hip_stream_copy", _i, _vp, _vp, _sz, _vp)
 
 
 class EinsplineError(RuntimeError):

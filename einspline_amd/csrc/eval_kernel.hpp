@@ -235,6 +235,47 @@ __device__ __forceinline__ void store_partial(A *dst, const A *r, int m) {
     if (e < m) st1(dst + e, r[e]);
 }
 
+// Make LDS writes of this wavefront visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One LDS staging area per (type, size); a kernel instantiates one of them.
+template <class A, int N>
+__device__ __forceinline__ A *lds_stage() {
+  __shared__ A buf[N];
+  return buf;
+}
+
+// Store a wavefront's contiguous output region of 64*K reals, where lane l
+// holds reals [l*K, (l+1)*K).  The values are transposed through the wave's
+// LDS slice so that every store instruction writes 64 consecutive 16-byte
+// pieces (1 KB) instead of 16 bytes at a 4*K-byte stride per lane.
+template <class A, int K>
+__device__ __forceinline__ void wave_store(A *dst, const A *r, A *lds) {
+  constexpr int PER16 = 16 / sizeof(A);
+  static_assert(K % PER16 == 0, "whole 16-byte pieces per lane");
+  typedef A v16 __attribute__((ext_vector_type(PER16)));
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < K / PER16; ++c) {
+    v16 q;
+#pragma unroll
+    for (int e = 0; e < PER16; ++e) q[e] = r[c * PER16 + e];
+    *(v16 *)(lds + lane * K + c * PER16) = q;
+  }
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < K / PER16; ++c) {
+    const int piece = c * 64 + lane;
+    const v16 q = *(const v16 *)(lds + piece * PER16);
+    __builtin_nontemporal_store(q, (v16 *)dst + piece);
+  }
+  wave_sync();
+}
+
 // ---------------------------------------------------------------------------
 // the kernel
 //
@@ -443,6 +484,21 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   const int m = P.nreal - e0;            // valid reals in this lane (may exceed V)
   const bool full = m >= V;
 
+  // One point per wavefront, 64 full chunks and aligned outputs: the
+  // wavefront's gradient and Hessian blocks are each one contiguous region,
+  // written through LDS with fully coalesced stores.
+  constexpr int PER16 = 16 / (int)sizeof(A);
+  constexpr bool STAGE = UNIF && MODE >= MODE_VG && ((3 * V) % PER16 == 0);
+  const int lane = threadIdx.x & 63;
+  const int wave_e0 = (ch - lane) * V;   // first real of the wavefront (UNIF)
+  bool staged = false;
+  A *lds = nullptr;
+  if constexpr (STAGE) {
+    staged = vec && (wave_e0 + 64 * V <= P.nreal);
+    constexpr int KMAX = (MODE == MODE_VGH ? 9 : 3) * V;
+    lds = lds_stage<A, 4 * KMAX * 64>() + (threadIdx.x >> 6) * (KMAX * 64);
+  }
+
   // values (and the Laplacian) keep the real index layout
   {
     A r[V];
@@ -466,8 +522,13 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         const int o = CPLX ? (e >> 1) * 6 + d * 2 + (e & 1) : e * 3 + d;
         r[o] = acc[1 + d][e];
       }
-    if (full) store_block<A, 3 * V>(og + 3 * e0, r, vec);
-    else store_partial<A, 3 * V>(og + 3 * e0, r, 3 * m);
+    if constexpr (STAGE) {
+      if (staged) wave_store<A, 3 * V>(og + 3 * wave_e0, r, lds);
+    }
+    if (!staged) {
+      if (full) store_block<A, 3 * V>(og + 3 * e0, r, vec);
+      else store_partial<A, 3 * V>(og + 3 * e0, r, 3 * m);
+    }
   }
   if constexpr (MODE == MODE_VGH) {
     // acc index of each Hessian entry, row-major 3x3 and packed upper triangle
@@ -482,8 +543,13 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           const int o = CPLX ? (e >> 1) * 12 + h * 2 + (e & 1) : e * 6 + h;
           r[o] = acc[pack6[h]][e];
         }
-      if (full) store_block<A, 6 * V>(oh + 6 * e0, r, vec);
-      else store_partial<A, 6 * V>(oh + 6 * e0, r, 6 * m);
+      if constexpr (STAGE) {
+        if (staged) wave_store<A, 6 * V>(oh + 6 * wave_e0, r, lds);
+      }
+      if (!staged) {
+        if (full) store_block<A, 6 * V>(oh + 6 * e0, r, vec);
+        else store_partial<A, 6 * V>(oh + 6 * e0, r, 6 * m);
+      }
     } else {
       A r[9 * V];
 #pragma unroll
@@ -493,8 +559,13 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           const int o = CPLX ? (e >> 1) * 18 + h * 2 + (e & 1) : e * 9 + h;
           r[o] = acc[full9[h]][e];
         }
-      if (full) store_block<A, 9 * V>(oh + 9 * e0, r, vec);
-      else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
+      if constexpr (STAGE) {
+        if (staged) wave_store<A, 9 * V>(oh + 9 * wave_e0, r, lds);
+      }
+      if (!staged) {
+        if (full) store_block<A, 9 * V>(oh + 9 * e0, r, vec);
+        else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
+      }
     }
   }
 }

@@ -307,6 +307,7 @@ void tune_save() {
 
 static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
                        void *v, void *g, void *h, int packed, void *stream);
+static int result_reals(const multi_UBspline_3d_hip *t);
 
 extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample, int *lb_out,
                                   int *blk_out, float *ms_out) {
@@ -690,6 +691,29 @@ POINT_API(s, float)
 POINT_API(d, double)
 POINT_API(c, complex_float)
 POINT_API(z, complex_double)
+
+// ---------------------------------------------------------------------------
+// plain copy kernel: the bandwidth ceiling and the counter calibration for
+// measured HBM traffic (a copy of B bytes reads B and writes B)
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) stream_copy_kernel(const u32x4 *__restrict__ src,
+                                                          u32x4 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+extern "C" int einspline_hip_stream_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  if (!dst || !src || bytes % 16) return fail(EINSPLINE_EINVAL, "copy needs 16-byte multiples");
+  const size_t n = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const u32x4 *)src, (u32x4 *)dst, n);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "stream copy launch");
+}
 
 // ---------------------------------------------------------------------------
 // minimal device runtime

@@ -181,6 +181,9 @@ int         einspline_hip_event_record(void *event, void *stream);
 int         einspline_hip_event_synchronize(void *event);
 /* Milliseconds between two recorded events; negative on error. */
 float       einspline_hip_event_elapsed_ms(void *start, void *stop);
+/* Plain device-to-device copy kernel (bytes a multiple of 16), used to
+ * measure the achievable HBM bandwidth and to calibrate traffic counters. */
+int         einspline_hip_stream_copy(void *dst_d, const void *src_d, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

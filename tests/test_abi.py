@@ -106,3 +106,28 @@ def test_oracle_is_not_linked_into_the_product():
     assert "oracle_" not in out
     deps = subprocess.run(["ldd", lib], capture_output=True, text=True).stdout
     assert "liboracle" not in deps
+
+
+def test_cxx_callers_of_reference_headers_link(tmp_path):
+    """A C++ object compiled against the reference's declarations (no
+    extern "C", src/multi_bspline_eval_s.h:86-109) links against the
+    library: the C++-mangled eval symbols exist."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(r'''
+#include <complex>
+#include "multi_bspline_structs.h"
+void eval_multi_UBspline_3d_s_vgh(multi_UBspline_3d_s *, double, double, double, float *, float *, float *);
+void eval_multi_UBspline_3d_z_vgl(multi_UBspline_3d_z *, double, double, double, std::complex<double> *,
+                                  std::complex<double> *, std::complex<double> *);
+int main(int argc, char **) {
+  if (argc > 5) {   /* never called: linking is the test */
+    eval_multi_UBspline_3d_s_vgh(0, 0, 0, 0, 0, 0, 0);
+    eval_multi_UBspline_3d_z_vgl(0, 0, 0, 0, 0, 0, 0);
+  }
+  return 0;
+}
+''')
+    libdir = os.path.join(ROOT, "einspline_amd")
+    exe = tmp_path / "caller"
+    subprocess.run(["g++", "-I", INC, str(src), "-L", libdir, "-leinspline", "-Wl,-rpath," + libdir,
+                    "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)], check=True)
