@@ -142,6 +142,10 @@ def headline(args):
     tab = ea.DeviceTable(host)
     if args.lane_bytes or args.block:
         tab.set_tile(args.lane_bytes, args.block)
+        tile = (args.lane_bytes, args.block)
+    else:
+        lb, blk, _ = tab.autotune("vgh", 8192)
+        tile = (lb, blk)
     rng = np.random.default_rng(99)
     pts = rng.random((P, 3))
     pos = ea.DeviceBuffer.from_array(pts)
@@ -171,7 +175,8 @@ def headline(args):
                                "points (256 walkers x 128 electrons), one launch",
                    "grid": [M, M, M], "orbitals": N, "points": P, "mode": "vgh",
                    "table": "f32", "hessian_layout": "packed6" if packed else "full9",
-                   "table_bytes": int(tab.bytes)},
+                   "table_bytes": int(tab.bytes), "tile_lane_bytes": tile[0],
+                   "tile_block": tile[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
@@ -210,8 +215,12 @@ def precision(args, res):
     rows = {}
     for name, st, cb in (("f32", ea.STORE_F32, 4), ("bf16", ea.STORE_BF16, 2), ("f16", ea.STORE_F16, 2)):
         tab = ea.DeviceTable(host, storage=st)
-        if name != "f32":
-            tab.set_tile(args.lane_bytes_16 or 0, 0)
+        if name != "f32" and args.lane_bytes_16:
+            tab.set_tile(args.lane_bytes_16, 0)
+            tile = (args.lane_bytes_16, 256)
+        else:
+            lb, blk, _ = tab.autotune("vgh", 16384)
+            tile = (lb, blk)
         outs = tab.alloc_outputs("vgh", P, args.packed)
         wall, kern = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 3, args.packed)
         # errors on the sample
@@ -230,7 +239,8 @@ def precision(args, res):
         rows[name] = {"evals_per_s": P * N / (wall * 1e-3), "kernel_ms": kern,
                       "achieved_GBs": P * N * bpe / (kern * 1e-3) / 1e9,
                       "frac_hbm": P * N * bpe / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                      "bytes_per_eval": bpe, "table_bytes": int(tab.bytes), "error_vs_f64": errs}
+                      "bytes_per_eval": bpe, "table_bytes": int(tab.bytes), "tile": list(tile),
+                      "error_vs_f64": errs}
         log(f"[bench] precision {name}: {kern:.3f} ms/launch, v max rel {errs['v']['max_rel']:.2e}")
         tab.destroy()
     rows["bf16_speedup_vs_f32"] = rows["f32"]["kernel_ms"] / rows["bf16"]["kernel_ms"]

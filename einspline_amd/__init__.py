@@ -130,6 +130,13 @@ for k, (S, BC, RT, _, _) in KINDS.items():
     _proto(f"eval_multi_multi_UBspline_3d_{k}_vgh_hip", _i, C.POINTER(multi_UBspline_3d_hip),
            _vp, _vp, _vp, _vp, _i)
 
+for k in "sdcz":
+    _proto(f"eval_multi_multi_UBspline_3d_{k}_vgl_hip", _i, C.POINTER(multi_UBspline_3d_hip),
+           _vp, _vp, _vp, _vp, _i, _i)
+_proto("eval_multi_multi_UBspline_3d_s_vgl_sign_hip", _i, C.POINTER(multi_UBspline_3d_hip),
+       _vp, _vp, _vp, _vp, _vp, _i, _i)
+_proto("eval_multi_multi_UBspline_3d_s_sign_hip", _i, C.POINTER(multi_UBspline_3d_hip),
+       _vp, _vp, _vp, _i)
 _proto("create_multi_UBspline_3d_s_hip_conv", C.POINTER(multi_UBspline_3d_hip), C.POINTER(multi_UBspline_3d_d))
 _proto("create_multi_UBspline_3d_c_hip_conv", C.POINTER(multi_UBspline_3d_hip), C.POINTER(multi_UBspline_3d_z))
 _proto("create_multi_UBspline_3d_s_hip_store", C.POINTER(multi_UBspline_3d_hip),

@@ -32,7 +32,9 @@ namespace esk {
 #define ESK_PREFETCH 1
 #endif
 
-enum { MODE_V = 0, MODE_VG = 1, MODE_VGL = 2, MODE_VGH = 3 };
+// MODE_QMC: value, lattice-transformed gradient and Laplacian in the layout
+// of the reference's CUDA vgl (src/multi_bspline_cuda_s_impl.h:566-734).
+enum { MODE_V = 0, MODE_VG = 1, MODE_VGL = 2, MODE_VGH = 3, MODE_QMC = 4 };
 
 // Storage tags for 16-bit tables (accumulated in float).
 struct bf16s { uint16_t b; };
@@ -56,6 +58,9 @@ struct Params {
   void *out_v, *out_g, *out_h;   // strided bases, or device arrays of per-point pointers
   long ld_v, ld_g, ld_h;     // per-point strides of strided outputs, in reals
   int hpacked;               // vgh: 1 -> 6 unique Hessian entries per orbital, 0 -> full 9
+  const void *linv;          // MODE_QMC: 3x3 row-major lattice matrix G (accumulation type)
+  const void *sign;          // optional per-point factor applied to every output (or null)
+  long row_stride;           // MODE_QMC: distance between gradient/Laplacian rows, in reals
 };
 
 // ---------------------------------------------------------------------------
@@ -340,6 +345,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   }
 
   // accumulators
+  constexpr bool HESS = MODE == MODE_VGH || MODE == MODE_QMC;
   constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7 : 10;
   A acc[NACC][V];
 #pragma unroll
@@ -416,7 +422,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           acc[6][e] = fmaA(ab, s2[e], acc[6][e]);    // d2/dz2
         }
       }
-      if constexpr (MODE == MODE_VGH) {
+      if constexpr (HESS) {
         const A d2ab = d2a[i] * b[j], dadb = da[i] * db[j], ad2b = a[i] * d2b[j];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
@@ -454,7 +460,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
       acc[4][e] = lx + ly + lz;
     }
   }
-  if constexpr (MODE == MODE_VGH) {
+  if constexpr (HESS) {
     const A sxx = dxi * dxi, sxy = dxi * dyi, sxz = dxi * dzi;
     const A syy = dyi * dyi, syz = dyi * dzi, szz = dzi * dzi;
 #pragma unroll
@@ -465,6 +471,40 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   }
 
   // ---- outputs ----
+  if constexpr (MODE == MODE_QMC) {
+    // value, G*grad and tr((G^T G) H) as rows of the caller's grad_lapl block
+    const A *L = (const A *)P.linv;
+    A G[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) G[q] = L[q];
+    A GG[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2)
+        GG[3 * r + c2] = G[0 * 3 + r] * G[0 * 3 + c2] + G[1 * 3 + r] * G[1 * 3 + c2] +
+                         G[2 * 3 + r] * G[2 * 3 + c2];
+    const A sg = P.sign ? ((const A *)P.sign)[p] : (A)1;
+    A *ov = ((A *const *)P.out_v)[p];
+    A *ogl = ((A *const *)P.out_g)[p];
+    const int e0 = ch * V;
+    const long rs = P.row_stride;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      if (e0 + e >= P.nreal) break;
+      const A g0 = acc[1][e], g1 = acc[2][e], g2 = acc[3][e];
+      const A h00 = acc[4][e], h01 = acc[5][e], h02 = acc[6][e];
+      const A h11 = acc[7][e], h12 = acc[8][e], h22 = acc[9][e];
+      const A lap = GG[0] * h00 + GG[3] * h01 + GG[6] * h02 + GG[1] * h01 + GG[4] * h11 +
+                    GG[7] * h12 + GG[2] * h02 + GG[5] * h12 + GG[8] * h22;
+      st1(ov + e0 + e, sg * acc[0][e]);
+      st1(ogl + e0 + e + 0 * rs, sg * (G[0] * g0 + G[1] * g1 + G[2] * g2));
+      st1(ogl + e0 + e + 1 * rs, sg * (G[3] * g0 + G[4] * g1 + G[5] * g2));
+      st1(ogl + e0 + e + 2 * rs, sg * (G[6] * g0 + G[7] * g1 + G[8] * g2));
+      st1(ogl + e0 + e + 3 * rs, sg * lap);
+    }
+    return;
+  }
   A *ov, *og = nullptr, *oh = nullptr;
   if constexpr (PTRS) {
     ov = ((A *const *)P.out_v)[p];
@@ -502,8 +542,9 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   // values (and the Laplacian) keep the real index layout
   {
     A r[V];
+    const A sg = (MODE == MODE_V && P.sign) ? ((const A *)P.sign)[p] : (A)1;
 #pragma unroll
-    for (int e = 0; e < V; ++e) r[e] = acc[0][e];
+    for (int e = 0; e < V; ++e) r[e] = sg * acc[0][e];
     if (full) store_block<A, V>(ov + e0, r, vec); else store_partial<A, V>(ov + e0, r, m);
   }
   if constexpr (MODE == MODE_VGL) {

@@ -29,10 +29,13 @@ hipError_t go_api(const Launch &L, const Params &P, hipStream_t s) {
   const long total = (long)P.npoints * P.chunks;
   dim3 block(L.block), grid((unsigned)((total + L.block - 1) / L.block));
   if (L.api == API_STRIDED) {
-    hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, double, false>), grid, block, 0, s, P);
-    return hipGetLastError();
+    if constexpr (MODE != MODE_QMC) {
+      hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, double, false>), grid, block, 0, s, P);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
   }
-  if constexpr (LB == 16 && (MODE == MODE_V || MODE == MODE_VGH)) {
+  if constexpr (LB == 16 && (MODE == MODE_V || MODE == MODE_VGH || MODE == MODE_QMC)) {
     if (L.api == API_PTRS_F32POS) {
       hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, float, true>), grid, block, 0, s, P);
       return hipGetLastError();
@@ -59,6 +62,9 @@ hipError_t go_mode(const Launch &L, const Params &P, hipStream_t s) {
     case MODE_VG:  return go_unif<S, LB, MODE_VG, CPLX>(L, P, s);
     case MODE_VGL: return go_unif<S, LB, MODE_VGL, CPLX>(L, P, s);
     case MODE_VGH: return go_unif<S, LB, MODE_VGH, CPLX>(L, P, s);
+    case MODE_QMC:
+      if constexpr (LB == 16 && sizeof(S) >= 4) return go_unif<S, LB, MODE_QMC, CPLX>(L, P, s);
+      break;
   }
   return hipErrorInvalidValue;
 }

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <map>
+#include <thread>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -47,42 +48,7 @@ int esk_hip_fail(hipError_t e, const char *what) {
 extern "C" const char *einspline_hip_last_error(void) { return g_err; }
 
 // ---------------------------------------------------------------------------
-// host-side conversion to the storage types (round to nearest even)
-
-// Round a finite non-negative double to a binary format with `mant` explicit
-// mantissa bits, exponent bias `bias` and largest biased exponent `emax`.
-// Returns the bit pattern without sign; sets *ovf on overflow to infinity.
-static uint32_t round_small(double a, int mant, int bias, int emax, int *ovf) {
-  if (a == 0.0) return 0;
-  int e;
-  frexp(a, &e);                               // a = f * 2^e, f in [0.5, 1)
-  int E = e - 1 + bias;                       // biased exponent of a
-  if (E <= 0) {                               // subnormal: quantum 2^(1-bias-mant)
-    double q = ldexp(a, mant + bias - 1);
-    return (uint32_t)nearbyint(q);            // may round up into the smallest normal
-  }
-  double q = ldexp(a, mant - (e - 1));        // in [2^mant, 2^(mant+1))
-  double r = nearbyint(q);
-  if (r == ldexp(1.0, mant + 1)) { r = ldexp(1.0, mant); ++E; }
-  if (E >= emax) { *ovf = 1; return (uint32_t)emax << mant; }
-  return ((uint32_t)E << mant) | ((uint32_t)r - (1u << mant));
-}
-
-static uint16_t to_bf16(double d, int *ovf) {
-  uint16_t sign = signbit(d) ? 0x8000 : 0;
-  double a = fabs(d);
-  if (isnan(a)) return sign | 0x7fc0;
-  if (isinf(a)) { *ovf = 1; return sign | 0x7f80; }
-  return sign | (uint16_t)round_small(a, 7, 127, 255, ovf);
-}
-
-static uint16_t to_f16(double d, int *ovf) {
-  uint16_t sign = signbit(d) ? 0x8000 : 0;
-  double a = fabs(d);
-  if (isnan(a)) return sign | 0x7e00;
-  if (isinf(a)) { *ovf = 1; return sign | 0x7c00; }
-  return sign | (uint16_t)round_small(a, 10, 15, 31, ovf);
-}
+// storage conversion (device side, round to nearest even)
 
 static size_t storage_bytes(einspline_storage s) {
   switch (s) {
@@ -94,10 +60,59 @@ static size_t storage_bytes(einspline_storage s) {
   return 0;
 }
 
-// ---------------------------------------------------------------------------
-// device tables
-
 namespace {
+
+// double -> float rounded to odd: exact when representable, otherwise the
+// truncated value with its last bit set.  Rounding that result to a format
+// with at least two fewer mantissa bits (bf16, f16) equals rounding the
+// double directly.
+__device__ __forceinline__ float f64_to_f32_odd(double d) {
+  float f = (float)d;
+  if ((double)f == d || f != f) return f;
+  unsigned u = __float_as_uint(f);
+  if (fabs((double)f) > fabs(d)) u -= 1;        // step the magnitude back toward zero
+  return __uint_as_float(u | 1u);
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f, int &ovf) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);   // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  if ((u & 0x7f800000u) == 0x7f800000u) ovf = 1;                             // rounded to inf
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ uint16_t f32_to_f16(float f, int &ovf) {
+  if (fabsf(f) >= 65520.0f) ovf = 1;            // beyond the largest half, rounds to inf
+  return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+// Copy `nrows` rows of `nreal` source reals (float or double) into table
+// rows of `row` storage elements starting at row r0, zero-padding each row.
+template <class Src>
+__global__ void __launch_bounds__(256) convert_rows(const Src *__restrict__ src, void *table,
+                                                   size_t r0, size_t nrows, int nreal, int row,
+                                                   int storage, int *ovf_flag) {
+  const size_t total = nrows * (size_t)row;
+  int ovf = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / row, q = i - r * row;
+    const double v = q < (size_t)nreal ? (double)src[r * nreal + q] : 0.0;
+    const size_t o = (r0 + r) * row + q;
+    switch (storage) {
+      case EINSPLINE_STORE_F32: ((float *)table)[o] = (float)v; break;
+      case EINSPLINE_STORE_F64: ((double *)table)[o] = v; break;
+      case EINSPLINE_STORE_BF16:
+        ((uint16_t *)table)[o] = f32_to_bf16(sizeof(Src) == 8 ? f64_to_f32_odd(v) : (float)v, ovf);
+        break;
+      case EINSPLINE_STORE_F16:
+        ((uint16_t *)table)[o] = f32_to_f16(sizeof(Src) == 8 ? f64_to_f32_odd(v) : (float)v, ovf);
+        break;
+    }
+  }
+  if (ovf) atomicOr(ovf_flag, 1);
+}
 
 struct TableExt {
   multi_UBspline_3d_hip pub;   // must stay first
@@ -105,11 +120,26 @@ struct TableExt {
   int block;                   // tile: work-items per block
 };
 
+// host memcpy split over a few threads (large chunks are memory-bound)
+void par_memcpy(void *dst, const void *src, size_t n) {
+  const size_t per = (size_t)8 << 20;
+  const int nt = (int)std::min<size_t>(8, (n + per - 1) / per);
+  if (nt <= 1) { memcpy(dst, src, n); return; }
+  std::vector<std::thread> th;
+  const size_t piece = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const size_t a = t * piece, b = std::min(n, a + piece);
+    if (a < b) th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, b - a); });
+  }
+  for (auto &x : th) x.join();
+}
+
 }  // namespace
 
-// Upload `src` (host table: Nx*Ny*Nz rows of `nreal` reals, float if
-// src_float else double, row stride `src_row` reals) into a new device
-// table of storage `st`.
+// Upload `src` (host table: Nx*Ny*Nz contiguous rows of `nreal` reals,
+// float if src_float else double) into a new device table of storage `st`.
+// Rows go through two pinned staging buffers: the host copy of one chunk
+// overlaps the transfer and on-device conversion of the previous one.
 static multi_UBspline_3d_hip *upload(const void *src, int src_float, size_t src_row,
                                      int Nx, int Ny, int Nz, int nsplines, int ncomp,
                                      const Ugrid &xg, const Ugrid &yg, const Ugrid &zg,
@@ -117,69 +147,80 @@ static multi_UBspline_3d_hip *upload(const void *src, int src_float, size_t src_
   const size_t eb = storage_bytes(st);
   if (!eb) { fail(EINSPLINE_EINVAL, "unknown storage type %d", (int)st); return nullptr; }
   const size_t nreal = (size_t)nsplines * ncomp;
+  if (src_row != nreal) { fail(EINSPLINE_EINVAL, "host rows must be contiguous"); return nullptr; }
   // pad rows to 32 bytes so every supported lane width divides them
   const size_t per32 = 32 / eb;
   const size_t row = (nreal + per32 - 1) / per32 * per32;
   const size_t nrows = (size_t)Nx * Ny * Nz;
   const size_t bytes = nrows * row * eb;
+  const size_t sb = src_float ? 4 : 8;
 
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) { hip_fail(e, "hipGetDevice"); return nullptr; }
-  void *d = nullptr;
+  const size_t chunk_bytes = (size_t)64 << 20;
+  const size_t rows_per = std::max<size_t>(1, std::min(nrows, chunk_bytes / (nreal * sb)));
+  const size_t stage = rows_per * nreal * sb;
+  void *d = nullptr, *dstage[2] = {nullptr, nullptr}, *hstage[2] = {nullptr, nullptr};
+  int *dflag = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = EINSPLINE_OK;
   e = hipMalloc(&d, bytes);
   if (e != hipSuccess) {
     fail(EINSPLINE_ENOMEM, "hipMalloc of %zu bytes for a spline table: %s", bytes,
          hipGetErrorString(e));
     return nullptr;
   }
-  // convert in pieces through a pinned staging buffer
-  const size_t stage_bytes = (size_t)64 << 20;
-  size_t rows_per = std::max<size_t>(1, stage_bytes / (row * eb));
-  rows_per = std::min(rows_per, nrows);
-  void *stage = nullptr;
-  e = hipHostMalloc(&stage, rows_per * row * eb, hipHostMallocDefault);
-  if (e != hipSuccess) {
-    hipFree(d);
-    fail(EINSPLINE_ENOMEM, "hipHostMalloc staging: %s", hipGetErrorString(e));
-    return nullptr;
+  if ((e = hipMalloc(&dstage[0], stage)) != hipSuccess || (e = hipMalloc(&dstage[1], stage)) != hipSuccess ||
+      (e = hipHostMalloc(&hstage[0], stage, hipHostMallocDefault)) != hipSuccess ||
+      (e = hipHostMalloc(&hstage[1], stage, hipHostMallocDefault)) != hipSuccess ||
+      (e = hipMalloc((void **)&dflag, sizeof(int))) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming)) != hipSuccess ||
+      (e = hipMemsetAsync(dflag, 0, sizeof(int), s)) != hipSuccess)
+    rc = fail(EINSPLINE_ENOMEM, "upload staging: %s", hipGetErrorString(e));
+  int k = 0;
+  for (size_t r0 = 0; rc == EINSPLINE_OK && r0 < nrows; r0 += rows_per, ++k) {
+    const int b = k & 1;
+    const size_t nr = std::min(rows_per, nrows - r0);
+    if (k >= 2 && (e = hipEventSynchronize(ev[b])) != hipSuccess) { rc = hip_fail(e, "upload"); break; }
+    par_memcpy(hstage[b], (const char *)src + r0 * nreal * sb, nr * nreal * sb);
+    e = hipMemcpyAsync(dstage[b], hstage[b], nr * nreal * sb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "upload copy"); break; }
+    const size_t total = nr * row;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
+    if (src_float)
+      hipLaunchKernelGGL(convert_rows<float>, dim3(blocks), dim3(256), 0, s, (const float *)dstage[b], d,
+                         r0, nr, (int)nreal, (int)row, (int)st, dflag);
+    else
+      hipLaunchKernelGGL(convert_rows<double>, dim3(blocks), dim3(256), 0, s, (const double *)dstage[b], d,
+                         r0, nr, (int)nreal, (int)row, (int)st, dflag);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(ev[b], s)) != hipSuccess) {
+      rc = hip_fail(e, "upload convert");
+      break;
+    }
   }
   int ovf = 0;
-  for (size_t r0 = 0; r0 < nrows; r0 += rows_per) {
-    const size_t nr = std::min(rows_per, nrows - r0);
-    for (long rr = 0; rr < (long)nr; ++rr) {
-      const size_t r = r0 + rr;
-      unsigned char *dst = (unsigned char *)stage + (size_t)rr * row * eb;
-      for (size_t q = 0; q < row; ++q) {
-        double v = 0.0;
-        if (q < nreal)
-          v = src_float ? (double)((const float *)src)[r * src_row + q]
-                        : ((const double *)src)[r * src_row + q];
-        switch (st) {
-          case EINSPLINE_STORE_F32: ((float *)dst)[q] = (float)v; break;
-          case EINSPLINE_STORE_F64: ((double *)dst)[q] = v; break;
-          case EINSPLINE_STORE_BF16: ((uint16_t *)dst)[q] = to_bf16(v, &ovf); break;
-          case EINSPLINE_STORE_F16: ((uint16_t *)dst)[q] = to_f16(v, &ovf); break;
-        }
-      }
-    }
-    if (ovf) break;
-    e = hipMemcpy((unsigned char *)d + r0 * row * eb, stage, nr * row * eb, hipMemcpyHostToDevice);
-    if (e != hipSuccess) break;
+  if (s && (e = hipStreamSynchronize(s)) != hipSuccess && rc == EINSPLINE_OK) rc = hip_fail(e, "upload");
+  if (rc == EINSPLINE_OK && (e = hipMemcpy(&ovf, dflag, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+    rc = hip_fail(e, "upload flag");
+  for (int b = 0; b < 2; ++b) {
+    if (dstage[b]) (void)hipFree(dstage[b]);
+    if (hstage[b]) (void)hipHostFree(hstage[b]);
+    if (ev[b]) (void)hipEventDestroy(ev[b]);
   }
-  hipHostFree(stage);
-  if (ovf) {
-    hipFree(d);
-    fail(EINSPLINE_ERANGE, "coefficient outside the range of the storage type");
-    return nullptr;
-  }
-  if (e != hipSuccess) {
-    hipFree(d);
-    hip_fail(e, "hipMemcpy table upload");
+  if (dflag) (void)hipFree(dflag);
+  if (s) (void)hipStreamDestroy(s);
+  if (rc == EINSPLINE_OK && ovf)
+    rc = fail(EINSPLINE_ERANGE, "coefficient outside the range of the storage type");
+  if (rc != EINSPLINE_OK) {
+    (void)hipFree(d);
     return nullptr;
   }
   TableExt *t = (TableExt *)calloc(1, sizeof(TableExt));
-  if (!t) { hipFree(d); fail(EINSPLINE_ENOMEM, "out of host memory"); return nullptr; }
+  if (!t) { (void)hipFree(d); fail(EINSPLINE_ENOMEM, "out of host memory"); return nullptr; }
   multi_UBspline_3d_hip *p = &t->pub;
   p->coefs = d;
   p->z_stride = (intptr_t)row;
@@ -489,7 +530,8 @@ extern "C" int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *t, const do
 
 // CUDA-style: per-point pointer arrays, packed Hessian, synchronous.
 static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, const void *pos,
-                    bool pos_f32, void **v, void **g, void **h, int num) {
+                    bool pos_f32, void **v, void **g, void **h, int num,
+                    const void *linv = nullptr, const void *sign = nullptr, int row_stride = 0) {
   if (t && t->tcode != want)
     return fail(EINSPLINE_EINVAL, "device table has the wrong type for this entry point");
   if (t && want == SINGLE_REAL && t->storage == EINSPLINE_STORE_F64)
@@ -499,8 +541,11 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
   int rc = esk_prepare(t, num, L, P);
   if (rc) return rc;
   if (num == 0) return EINSPLINE_OK;
-  if (!pos || !v || (mode == esk::MODE_VGH && (!g || !h)))
-    return fail(EINSPLINE_EINVAL, "NULL position or output pointer array");
+  if (!pos || !v || (mode == esk::MODE_VGH && (!g || !h)) || (mode == esk::MODE_QMC && (!g || !linv)))
+    return fail(EINSPLINE_EINVAL, "NULL position, Linv or output pointer array");
+  if (mode == esk::MODE_QMC && row_stride < t->num_splines)
+    return fail(EINSPLINE_EINVAL, "row_stride %d is smaller than num_splines %d", row_stride,
+                t->num_splines);
   L.lane_bytes = 16;            // the pointer-array kernels exist for 16-byte lanes
   if (t->row_reals * (int)storage_bytes(t->storage) % 16)
     return fail(EINSPLINE_EINVAL, "row not a multiple of 16 bytes");
@@ -510,6 +555,9 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
   P.pos = pos;
   P.out_v = v; P.out_g = g; P.out_h = h;
   P.hpacked = 1;
+  P.linv = linv;
+  P.sign = sign;
+  P.row_stride = (long)row_stride * (L.cplx ? 2 : 1);
   hipError_t e = dispatch(t, L, P, nullptr);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   e = hipDeviceSynchronize();
@@ -558,6 +606,52 @@ extern "C" int eval_multi_multi_UBspline_3d_z_vgh_hip(multi_UBspline_3d_z_hip *t
                                                       complex_double *hess[], int num) {
   return run_ptrs(t, DOUBLE_COMPLEX, esk::MODE_VGH, pos, false, (void **)vals, (void **)grads,
                   (void **)hess, num);
+}
+
+// QMC-shaped evaluation (src/multi_bspline_eval_cuda.h:67-79, 96-99, 112-115,
+// 131-134): value (optionally times a per-point sign), the gradient
+// transformed by the 3x3 lattice matrix Linv, and the Laplacian
+// tr((Linv^T Linv) H), written as rows grad_lapl[i] + d*row_stride (d = 0..2
+// gradient, 3 Laplacian; row_stride counts result elements).
+extern "C" int eval_multi_multi_UBspline_3d_s_sign_hip(multi_UBspline_3d_s_hip *t, float *pos,
+                                                       float *sign, float *vals[], int num) {
+  if (!sign) return fail(EINSPLINE_EINVAL, "NULL sign array");
+  return run_ptrs(t, SINGLE_REAL, esk::MODE_V, pos, true, (void **)vals, nullptr, nullptr, num,
+                  nullptr, sign);
+}
+extern "C" int eval_multi_multi_UBspline_3d_s_vgl_hip(multi_UBspline_3d_s_hip *t, float *pos,
+                                                      float *Linv, float *vals[],
+                                                      float *grad_lapl[], int num, int row_stride) {
+  return run_ptrs(t, SINGLE_REAL, esk::MODE_QMC, pos, true, (void **)vals, (void **)grad_lapl,
+                  nullptr, num, Linv, nullptr, row_stride);
+}
+extern "C" int eval_multi_multi_UBspline_3d_s_vgl_sign_hip(multi_UBspline_3d_s_hip *t, float *pos,
+                                                           float *sign, float *Linv, float *vals[],
+                                                           float *grad_lapl[], int num,
+                                                           int row_stride) {
+  if (!sign) return fail(EINSPLINE_EINVAL, "NULL sign array");
+  return run_ptrs(t, SINGLE_REAL, esk::MODE_QMC, pos, true, (void **)vals, (void **)grad_lapl,
+                  nullptr, num, Linv, sign, row_stride);
+}
+extern "C" int eval_multi_multi_UBspline_3d_d_vgl_hip(multi_UBspline_3d_d_hip *t, double *pos,
+                                                      double *Linv, double *vals[],
+                                                      double *grad_lapl[], int num, int row_stride) {
+  return run_ptrs(t, DOUBLE_REAL, esk::MODE_QMC, pos, false, (void **)vals, (void **)grad_lapl,
+                  nullptr, num, Linv, nullptr, row_stride);
+}
+extern "C" int eval_multi_multi_UBspline_3d_c_vgl_hip(multi_UBspline_3d_c_hip *t, float *pos,
+                                                      float *Linv, complex_float *vals[],
+                                                      complex_float *grad_lapl[], int num,
+                                                      int row_stride) {
+  return run_ptrs(t, SINGLE_COMPLEX, esk::MODE_QMC, pos, true, (void **)vals, (void **)grad_lapl,
+                  nullptr, num, Linv, nullptr, row_stride);
+}
+extern "C" int eval_multi_multi_UBspline_3d_z_vgl_hip(multi_UBspline_3d_z_hip *t, double *pos,
+                                                      double *Linv, complex_double *vals[],
+                                                      complex_double *grad_lapl[], int num,
+                                                      int row_stride) {
+  return run_ptrs(t, DOUBLE_COMPLEX, esk::MODE_QMC, pos, false, (void **)vals, (void **)grad_lapl,
+                  nullptr, num, Linv, nullptr, row_stride);
 }
 
 // ---------------------------------------------------------------------------

@@ -99,6 +99,8 @@ struct Table {
   einspline_storage storage = EINSPLINE_STORE_F32;
   size_t bytes = 0;                 // device bytes once admitted
   multi_UBspline_3d_hip *dev = nullptr;
+  bool loading = false;             // upload in progress on the uploader thread
+  size_t reserved = 0;              // bytes reserved for the upload in progress
   uint64_t last_use = 0;
   int inflight = 0;                 // requests of this table on the GPU
   int nsplines = 0;
@@ -139,8 +141,11 @@ struct einspline_server {
   std::unordered_map<uint64_t, Req *> live;
   std::vector<Table> tables;
   std::vector<Slot> slots;
-  std::thread worker;
+  std::thread worker, uploader;
+  std::deque<int> uploads;          // tables waiting for the uploader thread
+  std::condition_variable cv_upload;
   bool stop = false;
+  bool stop_uploader = false;       // set after the worker has drained
   int worker_error = 0;
   uint64_t next_id = 1, use_clock = 0;
   size_t budget = 0, used = 0;
@@ -149,6 +154,7 @@ struct einspline_server {
 
   int init(const einspline_server_config *c);
   void run();
+  void upload_loop();
   void poll_completions();
   bool launch_batch(Slot &s, std::unique_lock<std::mutex> &lk);
   int admit(int tid, std::unique_lock<std::mutex> &lk, const std::vector<char> &pinned);
@@ -212,6 +218,7 @@ int einspline_server::init(const einspline_server_config *c) {
       return esk_hip_fail(e, "server slot setup");
   }
   worker = std::thread([this] { run(); });
+  uploader = std::thread([this] { upload_loop(); });
   return EINSPLINE_OK;
 }
 
@@ -233,38 +240,83 @@ hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
   return exec;
 }
 
-// Make table `tid` resident.  Returns 1 admitted/resident, 0 deferred.
+// Make table `tid` resident.  Returns 1 when resident, 0 when deferred
+// (upload queued or in progress, or no room yet), -1 when it can never fit.
+// Uploads run on the uploader thread, so the worker keeps launching batches
+// for resident tables meanwhile.  Eviction frees idle tables only; it prefers
+// the smallest single table that makes enough room (least to re-upload),
+// then least-recently-used ones.
 int einspline_server::admit(int tid, std::unique_lock<std::mutex> &lk,
                             const std::vector<char> &pinned) {
+  (void)lk;
   Table &t = tables[tid];
   if (t.dev) return 1;
+  if (t.loading) return 0;
   const size_t need = predicted_bytes(t);
   if (need > budget) return -1;
   while (used + need > budget) {
-    // evict the least recently used table with nothing in flight
+    const size_t shortfall = used + need - budget;
     int victim = -1;
-    for (int i = 0; i < (int)tables.size(); ++i)
-      if (tables[i].dev && tables[i].inflight == 0 && !pinned[i] &&
-          (victim < 0 || tables[i].last_use < tables[victim].last_use))
+    for (int i = 0; i < (int)tables.size(); ++i) {
+      const Table &c = tables[i];
+      if (!c.dev || c.inflight || pinned[i] || c.bytes < shortfall) continue;
+      if (victim < 0 || c.bytes < tables[victim].bytes ||
+          (c.bytes == tables[victim].bytes && c.last_use < tables[victim].last_use))
         victim = i;
+    }
+    if (victim < 0)
+      for (int i = 0; i < (int)tables.size(); ++i) {
+        const Table &c = tables[i];
+        if (c.dev && !c.inflight && !pinned[i] &&
+            (victim < 0 || c.last_use < tables[victim].last_use))
+          victim = i;
+      }
     if (victim < 0) return 0;
     destroy_multi_UBspline_3d_hip(tables[victim].dev);
     tables[victim].dev = nullptr;
     used -= tables[victim].bytes;
     st.evictions++;
   }
-  used += need;                         // reserve before dropping the lock
-  lk.unlock();
-  multi_UBspline_3d_hip *d =
-      t.from_double ? create_multi_UBspline_3d_s_hip_store((const multi_UBspline_3d_d *)t.host, t.storage)
-                    : create_multi_UBspline_3d_s_hip((multi_UBspline_3d_s *)t.host);
-  lk.lock();
-  if (!d) { used -= need; return 0; }
-  t.dev = d;
-  t.bytes = d->bytes;
-  used = used - need + d->bytes;
-  st.admissions++;
-  return 1;
+  used += need;
+  t.reserved = need;
+  t.loading = true;
+  uploads.push_back(tid);
+  cv_upload.notify_one();
+  return 0;
+}
+
+void einspline_server::upload_loop() {
+  hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    cv_upload.wait(lk, [&] { return stop_uploader || !uploads.empty(); });
+    if (uploads.empty()) {
+      if (stop_uploader) return;
+      continue;
+    }
+    const int tid = uploads.front();
+    uploads.pop_front();
+    Table &t = tables[tid];
+    void *host = t.host;
+    const bool from_double = t.from_double;
+    const einspline_storage storage = t.storage;
+    lk.unlock();
+    multi_UBspline_3d_hip *d =
+        from_double ? create_multi_UBspline_3d_s_hip_store((const multi_UBspline_3d_d *)host, storage)
+                    : create_multi_UBspline_3d_s_hip((multi_UBspline_3d_s *)host);
+    lk.lock();
+    Table &u = tables[tid];
+    used -= u.reserved;
+    u.reserved = 0;
+    u.loading = false;
+    if (d) {
+      u.dev = d;
+      u.bytes = d->bytes;
+      used += d->bytes;
+      st.admissions++;
+    }
+    cv_work.notify_all();
+  }
 }
 
 void einspline_server::poll_completions() {
@@ -391,13 +443,14 @@ void einspline_server::run() {
     bool launched = false;
     if (free_slot && !pending.empty()) launched = launch_batch(*free_slot, lk);
     if (!launched) {
-      if (any_busy || (free_slot && !pending.empty())) {
+      if (any_busy) {
         // completions are polled: yield briefly without sleeping long
         lk.unlock();
         std::this_thread::yield();
         lk.lock();
       } else {
-        cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        // idle, or waiting for an upload: submit and the uploader notify
+        cv_work.wait_for(lk, std::chrono::milliseconds(1));
       }
     }
   }
@@ -424,6 +477,12 @@ extern "C" void einspline_server_destroy(einspline_server *s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop_uploader = true;
+  }
+  s->cv_upload.notify_all();
+  if (s->uploader.joinable()) s->uploader.join();
   for (Slot &sl : s->slots) {
     for (auto &kv : sl.graphs) hipGraphExecDestroy(kv.second);
     if (sl.stream) hipStreamDestroy(sl.stream);

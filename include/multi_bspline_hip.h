@@ -122,6 +122,31 @@ int eval_multi_multi_UBspline_3d_z_vgh_hip(multi_UBspline_3d_z_hip *spline, doub
                                            complex_double *vals_d[], complex_double *grads_d[],
                                            complex_double *hess_d[], int num);
 
+/* QMC-shaped outputs (src/multi_bspline_eval_cuda.h:67-79, 96-99, 112-115,
+ * 131-134): for point i, vals_d[i][n] is the value (times sign_d[i] for the
+ * _sign variants); grad_lapl_d[i] + d*row_stride, d = 0..2, holds the
+ * gradient transformed by the row-major 3x3 lattice matrix Linv_d
+ * (g'_d = sum_j Linv[3d+j] g_j), and grad_lapl_d[i] + 3*row_stride holds
+ * the Laplacian tr((Linv^T Linv) H).  row_stride counts result elements
+ * (complex ones for c/z) and must be >= num_splines.  Synchronous. */
+int eval_multi_multi_UBspline_3d_s_sign_hip(multi_UBspline_3d_s_hip *spline, float *pos_d,
+                                            float *sign_d, float *vals_d[], int num);
+int eval_multi_multi_UBspline_3d_s_vgl_hip(multi_UBspline_3d_s_hip *spline, float *pos_d,
+                                           float *Linv_d, float *vals_d[], float *grad_lapl_d[],
+                                           int num, int row_stride);
+int eval_multi_multi_UBspline_3d_s_vgl_sign_hip(multi_UBspline_3d_s_hip *spline, float *pos_d,
+                                                float *sign_d, float *Linv_d, float *vals_d[],
+                                                float *grad_lapl_d[], int num, int row_stride);
+int eval_multi_multi_UBspline_3d_d_vgl_hip(multi_UBspline_3d_d_hip *spline, double *pos_d,
+                                           double *Linv_d, double *vals_d[],
+                                           double *grad_lapl_d[], int num, int row_stride);
+int eval_multi_multi_UBspline_3d_c_vgl_hip(multi_UBspline_3d_c_hip *spline, float *pos_d,
+                                           float *Linv_d, complex_float *vals_d[],
+                                           complex_float *grad_lapl_d[], int num, int row_stride);
+int eval_multi_multi_UBspline_3d_z_vgl_hip(multi_UBspline_3d_z_hip *spline, double *pos_d,
+                                           double *Linv_d, complex_double *vals_d[],
+                                           complex_double *grad_lapl_d[], int num, int row_stride);
+
 /* ---- strided batched evaluation (main entry points) --------------------
  * Evaluates `num` points pos_d[3*i+{0,1,2}] (doubles, any storage type) in
  * one launch on `stream`.  Point i's results go to

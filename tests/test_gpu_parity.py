@@ -265,3 +265,107 @@ def test_autotune_picks_a_valid_tile_and_keeps_results(rng):
     tab2 = ea.DeviceTable(h)
     lb2, blk2, ms2 = tab2.autotune("vgh", 4096)
     assert (lb2, blk2) == (lb, blk) and ms2 < 0
+
+
+def _bf16_rne_from_f64(x):
+    """Round float64 values (normal bf16 range) to bf16 bit patterns, nearest even."""
+    b = np.asarray(x, np.float64).view(np.uint64)
+    sign = (b >> np.uint64(63)).astype(np.uint32)
+    lsb = (b >> np.uint64(45)) & np.uint64(1)
+    r = (b & np.uint64(0x7FFFFFFFFFFFFFFF)) + np.uint64((1 << 44) - 1) + lsb
+    e = (r >> np.uint64(52)).astype(np.int64) - 1023 + 127
+    m = ((r >> np.uint64(45)) & np.uint64(0x7F)).astype(np.uint32)
+    return ((sign << 15) | (e.astype(np.uint32) << 7) | m).astype(np.uint16)
+
+
+@pytest.mark.parametrize("storage", [ea.STORE_BF16, ea.STORE_F16, ea.STORE_F32])
+def test_storage_conversion_rounds_to_nearest_even(storage, rng):
+    """The device table holds each FP64 coefficient rounded once to the
+    storage type (no double rounding through float)."""
+    M, N = (4, 4, 4), 5
+    h = ea.HostSpline("d", M, (0, 0, 0), (1, 1, 1), N)
+    c = h.coefs()
+    vals = rng.uniform(0.5, 2.0, c.shape) * np.where(rng.random(c.shape) < 0.5, -1, 1)
+    # include values sitting just off a bf16 / f16 rounding midpoint
+    vals.flat[0] = 1.0 + 2.0**-8 + 2.0**-30      # just above a bf16 midpoint: rounds up
+    vals.flat[1] = 1.0 + 2.0**-8 - 2.0**-30      # just below: rounds down
+    vals.flat[2] = 1.0 + 2.0**-11 + 2.0**-40     # f16 midpoint + tiny: rounds up
+    c[...] = vals
+    tab = ea.DeviceTable(h, storage=storage)
+    t = tab.ptr.contents
+    rows = t.Nx * t.Ny * t.Nz
+    if storage == ea.STORE_F32:
+        raw = np.empty((rows, t.row_reals), np.float32)
+        ea.check(ea.lib.einspline_hip_memcpy_dtoh(raw.ctypes.data, t.coefs, raw.nbytes), "dtoh")
+        np.testing.assert_array_equal(raw[:, :N], vals.reshape(rows, N).astype(np.float32))
+        assert not raw[:, N:].any()
+        return
+    raw = np.empty((rows, t.row_reals), np.uint16)
+    ea.check(ea.lib.einspline_hip_memcpy_dtoh(raw.ctypes.data, t.coefs, raw.nbytes), "dtoh")
+    flat = vals.reshape(rows, N)
+    if storage == ea.STORE_BF16:
+        want = _bf16_rne_from_f64(flat)
+    else:
+        want = flat.astype(np.float16).view(np.uint16)
+    np.testing.assert_array_equal(raw[:, :N], want)
+    assert not raw[:, N:].any()
+
+
+def test_f16_overflow_is_reported(rng):
+    h = ea.HostSpline("d", (4, 4, 4), (0, 0, 0), (1, 1, 1), 2)
+    h.coefs()[...] = 1.0
+    h.coefs()[1, 1, 1, 0] = 70000.0
+    with pytest.raises(ea.EinsplineError) as e:
+        ea.DeviceTable(h, storage=ea.STORE_F16)
+    assert "range" in str(e.value)
+
+
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+def test_qmc_vgl_with_lattice_and_row_stride(kind, rng):
+    """eval_multi_multi_UBspline_3d_X_vgl_hip: value, Linv-transformed
+    gradient and tr((Linv^T Linv) H) as SoA rows row_stride apart
+    (src/multi_bspline_cuda_s_impl.h:566-734)."""
+    M, N, P = (7, 8, 6), 40, 33
+    h, o = make_pair(kind, M, N, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, P)
+    fpos = kind in "sc"
+    real = np.float32 if fpos else np.float64
+    pts_used = pts.astype(np.float32).astype(np.float64) if fpos else pts
+    pos = ea.DeviceBuffer.from_array(pts.astype(real))
+    G = np.array([[1.2, 0.1, -0.3], [0.0, 0.9, 0.2], [0.05, -0.1, 1.1]])
+    linv = ea.DeviceBuffer.from_array(G.astype(real).ravel())
+    nc = 2 if kind in "cz" else 1
+    es = np.dtype(real).itemsize * nc
+    rs = N + 5                                    # row stride > N
+    vb = ea.DeviceBuffer(P * N * es)
+    gb = ea.DeviceBuffer(P * 4 * rs * es)
+    gb.zero()
+    vp = ea.DeviceBuffer.from_array(np.array([vb.ptr + p * N * es for p in range(P)], np.uint64))
+    gp = ea.DeviceBuffer.from_array(np.array([gb.ptr + p * 4 * rs * es for p in range(P)], np.uint64))
+    f = getattr(ea.lib, f"eval_multi_multi_UBspline_3d_{kind}_vgl_hip")
+    ea.check(f(tab.ptr, pos.ptr, linv.ptr, vp.ptr, gp.ptr, P, rs), "vgl_hip")
+    cdt = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}[kind]
+    v = vb.download(real, (P, N * nc)).view(cdt)
+    gl = gb.download(real, (P, 4 * rs * nc)).view(cdt).reshape(P, 4, rs)
+    rv, rg, rH = o.eval_batch("vgh", pts_used)
+    Gd = G.astype(real).astype(np.float64)
+    g_ref = np.einsum("dj,pnj->pnd", Gd, rg)
+    lap_ref = np.einsum("ab,pnab->pn", Gd.T @ Gd, rH)
+    tol = TOL[kind] * 10
+    assert normwise(v, rv) < TOL[kind]
+    assert normwise(np.moveaxis(gl[:, :3, :N], 1, 2), g_ref) < tol
+    assert normwise(gl[:, 3, :N], lap_ref) < tol
+    assert not gl[:, :, N:].any()                 # padding between rows untouched
+    if kind == "s":
+        sign = ea.DeviceBuffer.from_array(np.where(rng.random(P) < 0.5, -1.0, 1.0).astype(np.float32))
+        sg = sign.download(np.float32, (P,))
+        f = ea.lib.eval_multi_multi_UBspline_3d_s_vgl_sign_hip
+        ea.check(f(tab.ptr, pos.ptr, sign.ptr, linv.ptr, vp.ptr, gp.ptr, P, rs), "vgl_sign_hip")
+        v2 = vb.download(real, (P, N))
+        gl2 = gb.download(real, (P, 4, rs))
+        np.testing.assert_array_equal(v2, sg[:, None] * v)
+        np.testing.assert_array_equal(gl2[:, :, :N], sg[:, None, None] * gl[:, :, :N])
+        ea.check(ea.lib.eval_multi_multi_UBspline_3d_s_sign_hip(tab.ptr, pos.ptr, sign.ptr, vp.ptr, P),
+                 "sign_hip")
+        np.testing.assert_array_equal(vb.download(real, (P, N)), sg[:, None] * v)
