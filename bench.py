@@ -10,11 +10,12 @@ over all points, inputs resident in HBM.
 Printed: one JSON line with evals/s (points x orbitals / s), the roofline
 of the kernel (algorithmic bytes per launch / measured launch time vs the
 8 TB/s HBM peak), the CPU baseline (the oracle's restatement of the
-reference's std evaluation, OpenMP over points on the host cores) and, with
---precision, the FP32 / BF16 / FP16 table comparison on the 96^3, 512-orbital
-workload with errors against the FP64 evaluation.
+reference's std evaluation, OpenMP over points on the host cores), the FP32 /
+BF16 / FP16 table comparison on the 96^3, 512-orbital workload with errors
+against the FP64 evaluation, and the serving form's p50/p99 latency under
+Poisson load (all under "extra").
 
-    python bench.py [--steps K] [--warmup W] [--precision] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-precision] [--no-serving] [--no-cpu]
 """
 import argparse
 import json
@@ -144,7 +145,7 @@ def headline(args):
         tab.set_tile(args.lane_bytes, args.block)
         tile = (args.lane_bytes, args.block)
     else:
-        lb, blk, _ = tab.autotune("vgh", 8192)
+        lb, blk, _ = tab.autotune("vgh", P)
         tile = (lb, blk)
     rng = np.random.default_rng(99)
     pts = rng.random((P, 3))
@@ -219,7 +220,7 @@ def precision(args, res):
             tab.set_tile(args.lane_bytes_16, 0)
             tile = (args.lane_bytes_16, 256)
         else:
-            lb, blk, _ = tab.autotune("vgh", 16384)
+            lb, blk, _ = tab.autotune("vgh", 32768)
             tile = (lb, blk)
         outs = tab.alloc_outputs("vgh", P, args.packed)
         wall, kern = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 3, args.packed)
@@ -317,11 +318,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", action="store_true", help="add the FP32/BF16/FP16 table comparison")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32/BF16/FP16 table comparison (96^3 x 512)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--serving", action="store_true", help="add the serving-form measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-form measurement")
     ap.add_argument("--no-extra", action="store_true", help="skip the packed-Hessian side measurement")
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
     ap.add_argument("--lane-bytes", type=int, default=0)
     ap.add_argument("--lane-bytes-16", type=int, default=0)
@@ -335,10 +337,24 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
     res = headline(args)
-    if args.precision:
+    if not args.no_precision:
         precision(args, res)
-    if args.serving:
+    if not args.no_serving:
         serving(args, res)
+    ex = res.get("extra", {})
+    summ = {}
+    if "precision" in ex:
+        summ["bf16_table_speedup_vs_f32"] = ex["precision"]["bf16_speedup_vs_f32"]
+        summ["bf16_vgh_evals_per_s_96c_n512"] = ex["precision"]["bf16"]["evals_per_s"]
+        summ["f32_vgh_evals_per_s_96c_n512"] = ex["precision"]["f32"]["evals_per_s"]
+    if "serving" in ex:
+        r = ex["serving"]["all_resident"]
+        summ["serving_p50_ms_at_50pct"] = r["load_50pct"]["p50_ms"]
+        summ["serving_p99_ms_at_50pct"] = r["load_50pct"]["p99_ms"]
+        summ["serving_p99_ms_at_90pct"] = r["load_90pct"]["p99_ms"]
+        summ["serving_capacity_evals_per_s"] = r["capacity_evals_per_s"]
+    if summ:
+        ex["summary"] = summ
     print(json.dumps(res), flush=True)
 
 

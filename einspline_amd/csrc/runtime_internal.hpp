@@ -9,3 +9,5 @@ int esk_fail(int code, const char *fmt, ...);
 int esk_hip_fail(hipError_t e, const char *what);
 // Kernel arguments and tile for `num` points of table `t` (outputs unset).
 int esk_prepare(const multi_UBspline_3d_hip *t, int num, esk::Launch &L, esk::Params &P);
+// Wrap device memory holding a table laid out as `meta` (takes ownership).
+multi_UBspline_3d_hip *esk_table_wrap(const multi_UBspline_3d_hip *meta, void *dcoefs);

@@ -101,6 +101,8 @@ struct Table {
   multi_UBspline_3d_hip *dev = nullptr;
   bool loading = false;             // upload in progress on the uploader thread
   size_t reserved = 0;              // bytes reserved for the upload in progress
+  void *pinned = nullptr;           // storage-format copy in pinned host memory
+  multi_UBspline_3d_hip meta{};     // geometry of the device table (valid with pinned)
   uint64_t last_use = 0;
   int inflight = 0;                 // requests of this table on the GPU
   int nsplines = 0;
@@ -300,10 +302,32 @@ void einspline_server::upload_loop() {
     void *host = t.host;
     const bool from_double = t.from_double;
     const einspline_storage storage = t.storage;
+    void *pinned = t.pinned;
+    const multi_UBspline_3d_hip meta = t.meta;
     lk.unlock();
-    multi_UBspline_3d_hip *d =
-        from_double ? create_multi_UBspline_3d_s_hip_store((const multi_UBspline_3d_d *)host, storage)
-                    : create_multi_UBspline_3d_s_hip((multi_UBspline_3d_s *)host);
+    multi_UBspline_3d_hip *d = nullptr;
+    void *new_pinned = nullptr;
+    if (pinned) {
+      // re-admission: the converted table comes straight from pinned memory
+      void *dc = nullptr;
+      if (hipMalloc(&dc, meta.bytes) == hipSuccess) {
+        if (hipMemcpy(dc, pinned, meta.bytes, hipMemcpyHostToDevice) == hipSuccess)
+          d = esk_table_wrap(&meta, dc);
+        if (!d) (void)hipFree(dc);
+      }
+    } else {
+      d = from_double ? create_multi_UBspline_3d_s_hip_store((const multi_UBspline_3d_d *)host, storage)
+                      : create_multi_UBspline_3d_s_hip((multi_UBspline_3d_s *)host);
+      // keep a storage-format host copy so later re-admissions skip conversion
+      if (d && hipHostMalloc(&new_pinned, d->bytes, hipHostMallocDefault) == hipSuccess) {
+        if (hipMemcpy(new_pinned, d->coefs, d->bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+          (void)hipHostFree(new_pinned);
+          new_pinned = nullptr;
+        }
+      } else {
+        new_pinned = nullptr;
+      }
+    }
     lk.lock();
     Table &u = tables[tid];
     used -= u.reserved;
@@ -314,6 +338,11 @@ void einspline_server::upload_loop() {
       u.bytes = d->bytes;
       used += d->bytes;
       st.admissions++;
+      if (new_pinned) {
+        u.pinned = new_pinned;
+        u.meta = *d;
+        u.meta.coefs = nullptr;
+      }
     }
     cv_work.notify_all();
   }
@@ -490,8 +519,10 @@ extern "C" void einspline_server_destroy(einspline_server *s) {
     if (sl.hbuf) hipHostFree(sl.hbuf);
     if (sl.dbuf) hipFree(sl.dbuf);
   }
-  for (Table &t : s->tables)
+  for (Table &t : s->tables) {
     if (t.dev) destroy_multi_UBspline_3d_hip(t.dev);
+    if (t.pinned) (void)hipHostFree(t.pinned);
+  }
   for (auto &kv : s->live) delete kv.second;
   delete s;
 }
