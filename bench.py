@@ -143,10 +143,10 @@ def headline(args):
     tab = ea.DeviceTable(host)
     if args.lane_bytes or args.block:
         tab.set_tile(args.lane_bytes, args.block)
-        tile = (args.lane_bytes, args.block)
+        tile = (args.lane_bytes, args.block, 0)
     else:
-        lb, blk, _ = tab.autotune("vgh", P)
-        tile = (lb, blk)
+        lb, blk, nt, _ = tab.autotune("vgh", P)
+        tile = (lb, blk, nt)
     rng = np.random.default_rng(99)
     pts = rng.random((P, 3))
     pos = ea.DeviceBuffer.from_array(pts)
@@ -177,7 +177,7 @@ def headline(args):
                    "grid": [M, M, M], "orbitals": N, "points": P, "mode": "vgh",
                    "table": "f32", "hessian_layout": "packed6" if packed else "full9",
                    "table_bytes": int(tab.bytes), "tile_lane_bytes": tile[0],
-                   "tile_block": tile[1]},
+                   "tile_block": tile[1], "store_policy": tile[2]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
@@ -218,10 +218,10 @@ def precision(args, res):
         tab = ea.DeviceTable(host, storage=st)
         if name != "f32" and args.lane_bytes_16:
             tab.set_tile(args.lane_bytes_16, 0)
-            tile = (args.lane_bytes_16, 256)
+            tile = (args.lane_bytes_16, 256, 0)
         else:
-            lb, blk, _ = tab.autotune("vgh", 32768)
-            tile = (lb, blk)
+            lb, blk, nt, _ = tab.autotune("vgh", 32768)
+            tile = (lb, blk, nt)
         outs = tab.alloc_outputs("vgh", P, args.packed)
         wall, kern = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 3, args.packed)
         # errors on the sample
@@ -313,6 +313,63 @@ def serving(args, res):
     res.setdefault("extra", {})["serving"] = out
 
 
+def complex_block(rng, M, count, nwaves=8, kmax=4):
+    """count complex orbitals sum_m c_m exp(2 pi i k_m.r), c_m ~ N(0,1) + i N(0,1)."""
+    x = np.arange(M) / M
+    out = np.empty((count, M, M, M), np.complex128)
+    for o in range(count):
+        K = rng.integers(-kmax, kmax + 1, size=(nwaves, 3))
+        c = rng.standard_normal(nwaves) + 1j * rng.standard_normal(nwaves)
+        acc = np.zeros((M, M, M), np.complex128)
+        for m in range(nwaves):
+            e = [np.exp(2j * np.pi * K[m, a] * x) for a in range(3)]
+            acc += c[m] * (e[0][:, None, None] * e[1][None, :, None] * e[2][None, None, :])
+        out[o] = acc
+    return out
+
+
+def other_configs(args, res):
+    """The other workload configurations, timed the same way: complex vgl on
+    a 64^3 table with 128 orbitals at 65536 points (z and c), and the FP64
+    32^3 x 64 vgh case at 4096 points."""
+    rows = {}
+    rng = np.random.default_rng(11)
+    hz = ea.HostSpline("z", (64, 64, 64), (0, 0, 0), (1, 1, 1), 128)
+    for first in range(0, 128, 32):
+        hz.set_block(first, complex_block(rng, 64, 32))
+    hc = ea.HostSpline("c", (64, 64, 64), (0, 0, 0), (1, 1, 1), 128)
+    hc.coefs()[...] = hz.coefs().astype(np.float32)     # same coefficients, FP32
+    pts = rng.random((65536, 3))
+    pos = ea.DeviceBuffer.from_array(pts)
+    for name, host, cb in (("z_vgl_64c_n128_p65536", hz, 16), ("c_vgl_64c_n128_p65536", hc, 8)):
+        tab = ea.DeviceTable(host)
+        lb, blk, nt, _ = tab.autotune("vgl", 32768)
+        outs = tab.alloc_outputs("vgl", 65536)
+        w, k = time_batches(tab, "vgl", pos, 65536, outs, max(5, args.steps // 2), 3)
+        bpe = 64 * cb + 5 * cb
+        ev = 65536 * 128
+        rows[name] = {"kernel_ms": k, "evals_per_s": ev / (w * 1e-3), "bytes_per_eval": bpe,
+                      "achieved_GBs": ev * bpe / (k * 1e-3) / 1e9,
+                      "frac_hbm": ev * bpe / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "table_bytes": int(tab.bytes), "tile": [lb, blk, nt]}
+        tab.destroy()
+    hd = ea.HostSpline("d", (32, 32, 32), (0, 0, 0), (1, 1, 1), 64)
+    hd.set_block(0, rng.uniform(-1, 1, (64, 32, 32, 32)))
+    tab = ea.DeviceTable(hd)
+    pts1 = rng.random((4096, 3))
+    pos1 = ea.DeviceBuffer.from_array(pts1)
+    outs = tab.alloc_outputs("vgh", 4096)
+    w, k = time_batches(tab, "vgh", pos1, 4096, outs, args.steps, 5)
+    bpe = 64 * 8 + 13 * 8
+    rows["d_vgh_32c_n64_p4096"] = {"kernel_ms": k, "evals_per_s": 4096 * 64 / (w * 1e-3),
+                                   "bytes_per_eval": bpe,
+                                   "achieved_GBs": 4096 * 64 * bpe / (k * 1e-3) / 1e9,
+                                   "note": "262144 evals per launch: too small to fill the GPU"}
+    for k2, r in rows.items():
+        log(f"[bench] {k2}: {r['kernel_ms']:.4f} ms/launch, {r['achieved_GBs']:.0f} GB/s")
+    res.setdefault("extra", {})["other_configs"] = rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,6 +379,7 @@ def main():
                     help="skip the FP32/BF16/FP16 table comparison (96^3 x 512)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-form measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the complex / FP64 configurations")
     ap.add_argument("--no-extra", action="store_true", help="skip the packed-Hessian side measurement")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
@@ -339,6 +397,8 @@ def main():
     res = headline(args)
     if not args.no_precision:
         precision(args, res)
+    if not args.no_configs:
+        other_configs(args, res)
     if not args.no_serving:
         serving(args, res)
     ex = res.get("extra", {})

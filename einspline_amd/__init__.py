@@ -146,7 +146,8 @@ _proto("destroy_Bspline", None, _vp)
 _proto("einspline_hip_invalidate", None, _vp)
 _proto("einspline_set_tile", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i)
 _proto("einspline_autotune", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i, C.POINTER(_i),
-       C.POINTER(_i), C.POINTER(C.c_float))
+       C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_float))
+_proto("einspline_set_store_policy", _i, C.POINTER(multi_UBspline_3d_hip), _i)
 _proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp)
 _proto("einspline_eval_vg_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp)
 _proto("einspline_eval_vgl_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp)
@@ -418,12 +419,17 @@ class DeviceTable:
     def set_tile(self, lane_bytes=0, block=0):
         check(lib.einspline_set_tile(self.ptr, int(lane_bytes), int(block)), "einspline_set_tile")
 
+    def set_store_policy(self, policy=0):
+        """0 staged + streaming (default), 1 staged + ordinary, 2 direct."""
+        check(lib.einspline_set_store_policy(self.ptr, int(policy)), "set_store_policy")
+
     def autotune(self, mode="vgh", sample_points=8192):
-        """Pick the fastest tile for `mode` on this table (cached per shape)."""
-        lb, blk, ms = _i(), _i(), C.c_float()
+        """Pick the fastest tile and store policy for `mode` on this table
+        (cached per shape).  Returns (lane_bytes, block, store_policy, ms)."""
+        lb, blk, nt, ms = _i(), _i(), _i(), C.c_float()
         check(lib.einspline_autotune(self.ptr, MODES.index(mode), int(sample_points), C.byref(lb),
-                                     C.byref(blk), C.byref(ms)), "einspline_autotune")
-        return lb.value, blk.value, ms.value
+                                     C.byref(blk), C.byref(nt), C.byref(ms)), "einspline_autotune")
+        return lb.value, blk.value, nt.value, ms.value
 
     def out_sizes(self, mode, packed=False):
         """Reals per point of (vals, grads, lapl/hess)."""

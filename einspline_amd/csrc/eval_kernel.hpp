@@ -32,6 +32,7 @@ namespace esk {
 #define ESK_PREFETCH 1
 #endif
 
+
 // MODE_QMC: value, lattice-transformed gradient and Laplacian in the layout
 // of the reference's CUDA vgl (src/multi_bspline_cuda_s_impl.h:566-734).
 enum { MODE_V = 0, MODE_VG = 1, MODE_VGL = 2, MODE_VGH = 3, MODE_QMC = 4 };
@@ -61,7 +62,14 @@ struct Params {
   const void *linv;          // MODE_QMC: 3x3 row-major lattice matrix G (accumulation type)
   const void *sign;          // optional per-point factor applied to every output (or null)
   long row_stride;           // MODE_QMC: distance between gradient/Laplacian rows, in reals
+  int store;                 // result store policy: STORE_STAGED_NT, STORE_STAGED, STORE_DIRECT
 };
+
+// Result store policies.  STAGED: gradient/Hessian blocks of a wavefront go
+// through LDS so every store instruction writes contiguous memory; NT:
+// streaming stores where an instruction writes whole lines; DIRECT: each
+// lane stores its own reals with ordinary stores.
+enum { STORE_STAGED_NT = 0, STORE_STAGED = 1, STORE_DIRECT = 2 };
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -207,23 +215,45 @@ template <int LB> struct Loader<f16s, LB> {
 
 // Streaming stores: results are written once and never re-read by the
 // kernel, so they bypass the caches that hold the coefficient table.
+// Result stores.  `nt` selects non-temporal (streaming) stores, which keep
+// the results from evicting the coefficient table from L2 and the Infinity
+// Cache.  They are used only where every store instruction of a wavefront
+// writes whole contiguous cache lines: a streaming store that covers part of
+// a line is far slower than a cached one.
 template <class T>
-__device__ __forceinline__ void st1(T *p, T v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_out(T v, T *p, bool nt) {
+  if (nt) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <class T>
+__device__ __forceinline__ void st1(T *p, T v) { *p = v; }
 
-// Store n = K reals held in r[] to dst (contiguous).  `vec` selects 16-byte
+// Store K reals held in r[] to dst (contiguous).  `vec` selects 16-byte
 // stores (dst must then be 16-byte aligned and K*sizeof(A) a multiple of 16).
 template <class A, int K>
-__device__ __forceinline__ void store_block(A *dst, const A *r, bool vec) {
+__device__ __forceinline__ void store_block(A *dst, const A *r, bool vec, bool nt) {
   constexpr int PER16 = 16 / sizeof(A);
+  if constexpr (K * sizeof(A) == 8) {
+    if (vec) {
+      typedef A v8 __attribute__((ext_vector_type(8 / sizeof(A))));
+      v8 q;
+#pragma unroll
+      for (int e = 0; e < K; ++e) q[e] = r[e];
+      st_out(q, (v8 *)dst, nt);       // 64 lanes x 8 bytes: whole lines
+      return;
+    }
+  }
   if constexpr ((K * sizeof(A)) % 16 == 0) {
     if (vec) {
       typedef A v16 __attribute__((ext_vector_type(PER16)));
+      // one piece per lane: consecutive lanes fill whole lines
+      const bool full_lines = nt && (K * sizeof(A) == 16);
 #pragma unroll
       for (int c = 0; c < K / PER16; ++c) {
         v16 q;
 #pragma unroll
         for (int e = 0; e < PER16; ++e) q[e] = r[c * PER16 + e];
-        __builtin_nontemporal_store(q, (v16 *)dst + c);
+        st_out(q, (v16 *)dst + c, full_lines);
       }
       return;
     }
@@ -259,24 +289,26 @@ __device__ __forceinline__ A *lds_stage() {
 // LDS slice so that every store instruction writes 64 consecutive 16-byte
 // pieces (1 KB) instead of 16 bytes at a 4*K-byte stride per lane.
 template <class A, int K>
-__device__ __forceinline__ void wave_store(A *dst, const A *r, A *lds) {
-  constexpr int PER16 = 16 / sizeof(A);
-  static_assert(K % PER16 == 0, "whole 16-byte pieces per lane");
-  typedef A v16 __attribute__((ext_vector_type(PER16)));
+__device__ __forceinline__ void wave_store(A *dst, const A *r, A *lds, bool nt) {
+  // largest piece (16 or 8 bytes) that divides a lane's K reals
+  constexpr int PB = (K * sizeof(A)) % 16 == 0 ? 16 : 8;
+  constexpr int PER = PB / (int)sizeof(A);
+  static_assert((K * sizeof(A)) % PB == 0, "whole pieces per lane");
+  typedef A vp __attribute__((ext_vector_type(PER)));
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int c = 0; c < K / PER16; ++c) {
-    v16 q;
+  for (int c = 0; c < K / PER; ++c) {
+    vp q;
 #pragma unroll
-    for (int e = 0; e < PER16; ++e) q[e] = r[c * PER16 + e];
-    *(v16 *)(lds + lane * K + c * PER16) = q;
+    for (int e = 0; e < PER; ++e) q[e] = r[c * PER + e];
+    *(vp *)(lds + lane * K + c * PER) = q;
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < K / PER16; ++c) {
+  for (int c = 0; c < K / PER; ++c) {
     const int piece = c * 64 + lane;
-    const v16 q = *(const v16 *)(lds + piece * PER16);
-    __builtin_nontemporal_store(q, (v16 *)dst + piece);
+    const vp q = *(const vp *)(lds + piece * PER);
+    st_out(q, (vp *)dst + piece, nt);
   }
   wave_sync();
 }
@@ -523,18 +555,18 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   const int e0 = ch * V;                 // first real of this lane
   const int m = P.nreal - e0;            // valid reals in this lane (may exceed V)
   const bool full = m >= V;
+  const bool nt = P.store == STORE_STAGED_NT;
 
   // One point per wavefront, 64 full chunks and aligned outputs: the
   // wavefront's gradient and Hessian blocks are each one contiguous region,
   // written through LDS with fully coalesced stores.
-  constexpr int PER16 = 16 / (int)sizeof(A);
-  constexpr bool STAGE = UNIF && MODE >= MODE_VG && ((3 * V) % PER16 == 0);
+  constexpr bool STAGE = UNIF && MODE >= MODE_VG && MODE != MODE_QMC && ((3 * V * sizeof(A)) % 8 == 0);
   const int lane = threadIdx.x & 63;
   const int wave_e0 = (ch - lane) * V;   // first real of the wavefront (UNIF)
   bool staged = false;
   A *lds = nullptr;
   if constexpr (STAGE) {
-    staged = vec && (wave_e0 + 64 * V <= P.nreal);
+    staged = vec && P.store != STORE_DIRECT && (wave_e0 + 64 * V <= P.nreal);
     constexpr int KMAX = (MODE == MODE_VGH ? 9 : 3) * V;
     lds = lds_stage<A, 4 * KMAX * 64>() + (threadIdx.x >> 6) * (KMAX * 64);
   }
@@ -545,13 +577,13 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
     const A sg = (MODE == MODE_V && P.sign) ? ((const A *)P.sign)[p] : (A)1;
 #pragma unroll
     for (int e = 0; e < V; ++e) r[e] = sg * acc[0][e];
-    if (full) store_block<A, V>(ov + e0, r, vec); else store_partial<A, V>(ov + e0, r, m);
+    if (full) store_block<A, V>(ov + e0, r, vec, nt); else store_partial<A, V>(ov + e0, r, m);
   }
   if constexpr (MODE == MODE_VGL) {
     A r[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) r[e] = acc[4][e];
-    if (full) store_block<A, V>(oh + e0, r, vec); else store_partial<A, V>(oh + e0, r, m);
+    if (full) store_block<A, V>(oh + e0, r, vec, nt); else store_partial<A, V>(oh + e0, r, m);
   }
   if constexpr (MODE >= MODE_VG) {
     // gradient: real orbital e -> [e][3]; complex orbital q -> [q][3][re,im]
@@ -564,10 +596,10 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         r[o] = acc[1 + d][e];
       }
     if constexpr (STAGE) {
-      if (staged) wave_store<A, 3 * V>(og + 3 * wave_e0, r, lds);
+      if (staged) wave_store<A, 3 * V>(og + 3 * wave_e0, r, lds, nt);
     }
     if (!staged) {
-      if (full) store_block<A, 3 * V>(og + 3 * e0, r, vec);
+      if (full) store_block<A, 3 * V>(og + 3 * e0, r, vec, nt);
       else store_partial<A, 3 * V>(og + 3 * e0, r, 3 * m);
     }
   }
@@ -585,10 +617,10 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           r[o] = acc[pack6[h]][e];
         }
       if constexpr (STAGE) {
-        if (staged) wave_store<A, 6 * V>(oh + 6 * wave_e0, r, lds);
+        if (staged) wave_store<A, 6 * V>(oh + 6 * wave_e0, r, lds, nt);
       }
       if (!staged) {
-        if (full) store_block<A, 6 * V>(oh + 6 * e0, r, vec);
+        if (full) store_block<A, 6 * V>(oh + 6 * e0, r, vec, nt);
         else store_partial<A, 6 * V>(oh + 6 * e0, r, 6 * m);
       }
     } else {
@@ -601,10 +633,10 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           r[o] = acc[full9[h]][e];
         }
       if constexpr (STAGE) {
-        if (staged) wave_store<A, 9 * V>(oh + 9 * wave_e0, r, lds);
+        if (staged) wave_store<A, 9 * V>(oh + 9 * wave_e0, r, lds, nt);
       }
       if (!staged) {
-        if (full) store_block<A, 9 * V>(oh + 9 * e0, r, vec);
+        if (full) store_block<A, 9 * V>(oh + 9 * e0, r, vec, nt);
         else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
       }
     }

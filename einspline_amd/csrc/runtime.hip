@@ -116,8 +116,9 @@ __global__ void __launch_bounds__(256) convert_rows(const Src *__restrict__ src,
 
 struct TableExt {
   multi_UBspline_3d_hip pub;   // must stay first
-  int lane_bytes;              // tile: bytes per lane per row
-  int block;                   // tile: work-items per block
+  int lane_bytes;              // tile: bytes per lane per row (0 = default)
+  int block;                   // tile: work-items per block (0 = default)
+  int store_policy;            // esk::STORE_* for result stores
 };
 
 // host memcpy split over a few threads (large chunks are memory-bound)
@@ -236,6 +237,7 @@ static multi_UBspline_3d_hip *upload(const void *src, int src_float, size_t src_
   p->device = dev;
   t->lane_bytes = 0;
   t->block = 0;
+  t->store_policy = esk::STORE_STAGED_NT;
   return p;
 }
 
@@ -294,7 +296,7 @@ extern "C" multi_UBspline_3d_z_hip *create_multi_UBspline_3d_z_hip(multi_UBsplin
 
 extern "C" void destroy_multi_UBspline_3d_hip(multi_UBspline_3d_hip *t) {
   if (!t) return;
-  if (t->coefs) hipFree(t->coefs);
+  if (t->coefs) (void)hipFree(t->coefs);
   free((TableExt *)t);
 }
 
@@ -316,6 +318,13 @@ extern "C" int einspline_set_tile(multi_UBspline_3d_hip *t, int lane_bytes, int 
   return EINSPLINE_OK;
 }
 
+extern "C" int einspline_set_store_policy(multi_UBspline_3d_hip *t, int policy) {
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
+  if (policy < 0 || policy > 2) return fail(EINSPLINE_EINVAL, "store policy must be 0, 1 or 2");
+  ((TableExt *)t)->store_policy = policy;
+  return EINSPLINE_OK;
+}
+
 // ---------------------------------------------------------------------------
 // autotuner
 
@@ -327,7 +336,10 @@ struct TuneKey {
     return std::tie(storage, tcode, mode, row_bytes) < std::tie(o.storage, o.tcode, o.mode, o.row_bytes);
   }
 };
-std::map<TuneKey, std::pair<int, int>> g_tune;
+struct TuneVal {
+  int lane_bytes, block, store;
+};
+std::map<TuneKey, TuneVal> g_tune;
 bool g_tune_loaded = false;
 
 void tune_load() {
@@ -338,9 +350,10 @@ void tune_load() {
   FILE *fp = fopen(f, "r");
   if (!fp) return;
   TuneKey k;
-  int lb, blk;
-  while (fscanf(fp, "%d %d %d %d %d %d", &k.storage, &k.tcode, &k.mode, &k.row_bytes, &lb, &blk) == 6)
-    g_tune[k] = {lb, blk};
+  TuneVal v;
+  while (fscanf(fp, "%d %d %d %d %d %d %d", &k.storage, &k.tcode, &k.mode, &k.row_bytes,
+                &v.lane_bytes, &v.block, &v.store) == 7)
+    g_tune[k] = v;
   fclose(fp);
 }
 
@@ -350,8 +363,8 @@ void tune_save() {
   FILE *fp = fopen(f, "w");
   if (!fp) return;
   for (auto &kv : g_tune)
-    fprintf(fp, "%d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
-            kv.first.row_bytes, kv.second.first, kv.second.second);
+    fprintf(fp, "%d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
+            kv.first.row_bytes, kv.second.lane_bytes, kv.second.block, kv.second.store);
   fclose(fp);
 }
 }  // namespace
@@ -361,7 +374,8 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
 static int result_reals(const multi_UBspline_3d_hip *t);
 
 extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample, int *lb_out,
-                                  int *blk_out, float *ms_out) {
+                                  int *blk_out, int *nt_out, float *ms_out) {
+  // nt_out receives the chosen store policy
   if (!t || mode < 0 || mode > 3) return fail(EINSPLINE_EINVAL, "bad autotune arguments");
   if (sample <= 0) sample = 8192;
   const int eb = (int)storage_bytes(t->storage);
@@ -371,9 +385,11 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
     tune_load();
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      einspline_set_tile(t, it->second.first, it->second.second);
-      if (lb_out) *lb_out = it->second.first;
-      if (blk_out) *blk_out = it->second.second;
+      einspline_set_tile(t, it->second.lane_bytes, it->second.block);
+      einspline_set_store_policy(t, it->second.store);
+      if (lb_out) *lb_out = it->second.lane_bytes;
+      if (blk_out) *blk_out = it->second.block;
+      if (nt_out) *nt_out = it->second.store;
       if (ms_out) *ms_out = -1.0f;
       return EINSPLINE_OK;
     }
@@ -404,18 +420,20 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
       (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
     rc = hip_fail(e, "autotune setup");
   TableExt *ext = (TableExt *)t;
-  const int old_lb = ext->lane_bytes, old_blk = ext->block;
-  int best_lb = 16, best_blk = 256;
+  const int old_lb = ext->lane_bytes, old_blk = ext->block, old_store = ext->store_policy;
+  int best_lb = 16, best_blk = 256, best_nt = esk::STORE_STAGED_NT;
   float best = -1.0f, default_ms = -1.0f;
   const int lbs_f64[] = {16, 32}, lbs_other[] = {8, 16, 32};
   const int *lbs = t->storage == EINSPLINE_STORE_F64 ? lbs_f64 : lbs_other;
   const int nlb = t->storage == EINSPLINE_STORE_F64 ? 2 : 3;
   const int blks[] = {64, 128, 256};
+  for (int nt = 0; nt <= 2; ++nt)
   for (int a = 0; rc == EINSPLINE_OK && a < nlb; ++a) {
     if ((t->row_reals * eb) % lbs[a]) continue;
     for (int b = 0; rc == EINSPLINE_OK && b < 3; ++b) {
       ext->lane_bytes = lbs[a];
       ext->block = blks[b];
+      ext->store_policy = nt;
       auto launch = [&]() {
         return run_strided(t, mode, (const double *)dp, sample, dv, dg, dh, 0, st);
       };
@@ -433,34 +451,39 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
         if (ms < 0.0f || t_ms < ms) ms = t_ms;
       }
       if (rc != EINSPLINE_OK) break;
-      if (lbs[a] == 16 && blks[b] == 256) default_ms = ms;
-      if (best < 0.0f || ms < best) { best = ms; best_lb = lbs[a]; best_blk = blks[b]; }
+      if (lbs[a] == 16 && blks[b] == 256 && nt == esk::STORE_STAGED_NT) default_ms = ms;
+      if (best < 0.0f || ms < best) { best = ms; best_lb = lbs[a]; best_blk = blks[b]; best_nt = nt; }
     }
   }
   ext->lane_bytes = old_lb;
   ext->block = old_blk;
-  // keep the default tile (16 B x 256) unless the winner is at least 3% faster
+  ext->store_policy = old_store;
+  // keep the default tile (16 B x 256, streaming stores) unless the winner is
+  // at least 3% faster
   if (default_ms > 0.0f && best > 0.97f * default_ms) {
     best = default_ms;
     best_lb = 16;
     best_blk = 256;
+    best_nt = esk::STORE_STAGED_NT;
   }
-  if (dp) hipFree(dp);
-  if (dv) hipFree(dv);
-  if (dg) hipFree(dg);
-  if (dh) hipFree(dh);
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  if (st) hipStreamDestroy(st);
+  if (dp) (void)hipFree(dp);
+  if (dv) (void)hipFree(dv);
+  if (dg) (void)hipFree(dg);
+  if (dh) (void)hipFree(dh);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
   if (rc != EINSPLINE_OK) return rc;
   einspline_set_tile(t, best_lb, best_blk);
+  einspline_set_store_policy(t, best_nt);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = {best_lb, best_blk};
+    g_tune[key] = {best_lb, best_blk, best_nt};
     tune_save();
   }
   if (lb_out) *lb_out = best_lb;
   if (blk_out) *blk_out = best_blk;
+  if (nt_out) *nt_out = best_nt;
   if (ms_out) *ms_out = best;
   return EINSPLINE_OK;
 }
@@ -495,6 +518,7 @@ int esk_prepare(const multi_UBspline_3d_hip *t, int num, Launch &L, Params &P) {
   P.chunks = row_bytes / lb;
   P.nreal = result_reals(t);
   P.npoints = num;
+  P.store = x->store_policy;
   if ((long long)num * P.chunks >= (1ll << 31))
     return fail(EINSPLINE_EINVAL, "batch too large: points * lanes per point must be < 2^31");
   return EINSPLINE_OK;
@@ -704,9 +728,9 @@ int stage_reserve(size_t bytes) {
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   if (g_stage.device != dev) {
-    if (g_stage.dev) hipFree(g_stage.dev);
-    if (g_stage.host) hipHostFree(g_stage.host);
-    if (g_stage.stream) hipStreamDestroy(g_stage.stream);
+    if (g_stage.dev) (void)hipFree(g_stage.dev);
+    if (g_stage.host) (void)hipHostFree(g_stage.host);
+    if (g_stage.stream) (void)hipStreamDestroy(g_stage.stream);
     g_stage.dev = g_stage.host = nullptr;
     g_stage.bytes = 0;
     g_stage.stream = nullptr;
@@ -717,8 +741,8 @@ int stage_reserve(size_t bytes) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
   }
   if (g_stage.bytes >= bytes) return EINSPLINE_OK;
-  if (g_stage.dev) hipFree(g_stage.dev);
-  if (g_stage.host) hipHostFree(g_stage.host);
+  if (g_stage.dev) (void)hipFree(g_stage.dev);
+  if (g_stage.host) (void)hipHostFree(g_stage.host);
   g_stage.dev = g_stage.host = nullptr;
   g_stage.bytes = 0;
   e = hipMalloc(&g_stage.dev, bytes);

@@ -236,7 +236,7 @@ hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
       hipLaunchKernelGGL(group_kernel(g), dim3(blocks[g]), dim3(256), 0, s.stream, d, g);
   if (hipStreamEndCapture(s.stream, &graph) != hipSuccess) return nullptr;
   hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
+  (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return nullptr;
   s.graphs[mask] = exec;
   return exec;
@@ -288,7 +288,7 @@ int einspline_server::admit(int tid, std::unique_lock<std::mutex> &lk,
 }
 
 void einspline_server::upload_loop() {
-  hipSetDevice(device);
+  (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
     cv_upload.wait(lk, [&] { return stop_uploader || !uploads.empty(); });
@@ -458,7 +458,7 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
 }
 
 void einspline_server::run() {
-  hipSetDevice(device);
+  (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
     poll_completions();
@@ -513,11 +513,11 @@ extern "C" void einspline_server_destroy(einspline_server *s) {
   s->cv_upload.notify_all();
   if (s->uploader.joinable()) s->uploader.join();
   for (Slot &sl : s->slots) {
-    for (auto &kv : sl.graphs) hipGraphExecDestroy(kv.second);
-    if (sl.stream) hipStreamDestroy(sl.stream);
-    if (sl.done) hipEventDestroy(sl.done);
-    if (sl.hbuf) hipHostFree(sl.hbuf);
-    if (sl.dbuf) hipFree(sl.dbuf);
+    for (auto &kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.hbuf) (void)hipHostFree(sl.hbuf);
+    if (sl.dbuf) (void)hipFree(sl.dbuf);
   }
   for (Table &t : s->tables) {
     if (t.dev) destroy_multi_UBspline_3d_hip(t.dev);

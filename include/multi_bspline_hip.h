@@ -173,14 +173,24 @@ int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *spline, const double *
  * launches on `spline`; returns EINSPLINE_EINVAL for unsupported shapes. */
 int einspline_set_tile(multi_UBspline_3d_hip *spline, int lane_bytes, int block_threads);
 
+/* Result store policy (results are identical under all three):
+ *   0 (default) gradient/Hessian blocks are transposed through LDS so each
+ *     wavefront store writes contiguous memory, with streaming
+ *     (non-temporal) stores that keep outputs from evicting the table;
+ *   1 the same transposition with ordinary stores;
+ *   2 each lane stores its own results with ordinary stores. */
+int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
+
 /* Autotuner (no reference counterpart): times every supported tile shape
- * (lane_bytes x block_threads) of `mode` (0 v, 1 vg, 2 vgl, 3 vgh) on
+ * (lane_bytes x block_threads x store policy) of `mode` (0 v, 1 vg, 2 vgl, 3 vgh) on
  * `sample_points` uniform random points of the table's box, applies the
- * fastest with einspline_set_tile and returns it.  Results are cached per
+ * fastest with einspline_set_tile / einspline_set_store_policy and
+ * returns it.  Results are cached per
  * (storage, type, mode, row bytes) for the process, and in the text file
  * named by $EINSPLINE_TUNE_CACHE when set.  Synchronises the device. */
 int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_points,
-                       int *lane_bytes, int *block_threads, float *best_ms);
+                       int *lane_bytes, int *block_threads, int *store_policy,
+                       float *best_ms);
 
 /* ---- single-point path support ----------------------------------------
  * The single-point eval_multi_UBspline_3d_* functions keep a device mirror

@@ -192,12 +192,15 @@ def test_tiles_are_bitwise_identical(kind, lbs, rng):
     tab = ea.DeviceTable(h)
     pts = points(rng, 77)
     base = gpu_eval(tab, "vgh", pts)
-    for lb in lbs:
-        for blk in (64, 128, 256):
-            tab.set_tile(lb, blk)
-            for a, b in zip(gpu_eval(tab, "vgh", pts), base):
-                np.testing.assert_array_equal(a, b)
+    for pol in (0, 1, 2):
+        tab.set_store_policy(pol)
+        for lb in lbs:
+            for blk in (64, 128, 256):
+                tab.set_tile(lb, blk)
+                for a, b in zip(gpu_eval(tab, "vgh", pts), base):
+                    np.testing.assert_array_equal(a, b)
     tab.set_tile(0, 0)
+    tab.set_store_policy(0)
 
 
 def test_results_independent_of_batch(rng):
@@ -256,15 +259,15 @@ def test_autotune_picks_a_valid_tile_and_keeps_results(rng):
     tab = ea.DeviceTable(h)
     pts = points(rng, 200)
     before = gpu_eval(tab, "vgh", pts)
-    lb, blk, ms = tab.autotune("vgh", 4096)
-    assert lb in (8, 16, 32) and blk in (64, 128, 256) and ms > 0
+    lb, blk, nt, ms = tab.autotune("vgh", 4096)
+    assert lb in (8, 16, 32) and blk in (64, 128, 256) and nt in (0, 1, 2) and ms > 0
     after = gpu_eval(tab, "vgh", pts)
     for a, b in zip(after, before):
         np.testing.assert_array_equal(a, b)
     # second call on the same shape hits the cache
     tab2 = ea.DeviceTable(h)
-    lb2, blk2, ms2 = tab2.autotune("vgh", 4096)
-    assert (lb2, blk2) == (lb, blk) and ms2 < 0
+    lb2, blk2, nt2, ms2 = tab2.autotune("vgh", 4096)
+    assert (lb2, blk2, nt2) == (lb, blk, nt) and ms2 < 0
 
 
 def _bf16_rne_from_f64(x):
