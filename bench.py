@@ -143,7 +143,8 @@ def headline(args):
     tab = ea.DeviceTable(host)
     if args.lane_bytes or args.block:
         tab.set_tile(args.lane_bytes, args.block)
-        tile = (args.lane_bytes, args.block, 0)
+        tab.set_store_policy(args.store_policy)
+        tile = (args.lane_bytes, args.block, args.store_policy)
     else:
         lb, blk, nt, _ = tab.autotune("vgh", P)
         tile = (lb, blk, nt)
@@ -187,6 +188,18 @@ def headline(args):
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, host cores)")
         res["cpu_baseline"] = cpu_baseline("s", host, pts, "vgh")
+    # the drop-in single-point API: one launch + result copy per call
+    if not args.no_extra:
+        x = pts[:200]
+        host.eval("vgh", *x[0])                 # builds the device mirror
+        t = []
+        for p in x:
+            t0 = time.perf_counter()
+            host.eval("vgh", *p)
+            t.append(time.perf_counter() - t0)
+        res.setdefault("extra", {})["single_point_vgh_call_us"] = {
+            "median": float(np.median(t) * 1e6), "p99": float(np.percentile(t, 99) * 1e6),
+            "note": "eval_multi_UBspline_3d_s_vgh through ctypes: H2D position, launch, D2H of 256 x 13 floats"}
     # layout variant for the record
     if not packed and not args.no_extra:
         outs_p = tab.alloc_outputs("vgh", P, True)
@@ -386,6 +399,8 @@ def main():
     ap.add_argument("--lane-bytes", type=int, default=0)
     ap.add_argument("--lane-bytes-16", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--store-policy", type=int, default=0,
+                    help="with --lane-bytes/--block: 0 staged+streaming, 1 staged, 2 direct")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the spline evaluator is a single-GPU kernel "

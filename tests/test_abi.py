@@ -131,3 +131,21 @@ int main(int argc, char **) {
     exe = tmp_path / "caller"
     subprocess.run(["g++", "-I", INC, str(src), "-L", libdir, "-leinspline", "-Wl,-rpath," + libdir,
                     "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)], check=True)
+
+
+def test_no_silent_cpu_fallback_without_a_gpu():
+    """Without a visible GPU the product fails loudly: device tables cannot be
+    created and the batched entry points return error codes (there is no CPU
+    evaluation path in the library)."""
+    import numpy as np
+
+    import einspline_amd as ea
+    if ea.device_count() > 0:
+        pytest.skip("a GPU is visible; this checks the no-GPU behaviour")
+    h = ea.HostSpline("s", (5, 5, 5), (0, 0, 0), (1, 1, 1), 3)
+    h.set_block(0, np.random.default_rng(0).uniform(-1, 1, (3, 5, 5, 5)))
+    with pytest.raises(ea.EinsplineError) as e:
+        ea.DeviceTable(h)
+    assert "hip" in str(e.value).lower()
+    assert ea.lib.einspline_eval_vgh_batch(None, None, 1, None, None, None, 0, None) == ea.E_INVAL
+    assert ea.lib.einspline_hip_malloc(16) is None

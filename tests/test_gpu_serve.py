@@ -125,3 +125,39 @@ def test_load_generator_smoke(rng):
     st = srv.stats()
     assert st["completed"] == st["submitted"]
     srv.close()
+
+
+def test_concurrent_submitters(rng):
+    """Several host threads submit and wait at once (ctypes drops the GIL)."""
+    import threading
+
+    hosts = tables(rng)
+    storages = [ea.STORE_F32, ea.STORE_BF16, ea.STORE_F32]
+    srv = Server(slots=4, max_batch_points=4096, max_batch_requests=64)
+    ids = [srv.register(h, s) for h, s in zip(hosts, storages)]
+    errors = []
+
+    def worker(seed):
+        r = np.random.default_rng(seed)
+        try:
+            for _ in range(25):
+                k = int(r.integers(0, 3))
+                n = int(r.integers(1, 200))
+                pts = r.random((n, 3))
+                N = hosts[k].N
+                bufs = [ea.DeviceBuffer(n * N * c * 4) for c in (1, 3, 9)]
+                srv.wait(srv.submit(ids[k], pts, *bufs))
+                v = bufs[0].download(np.float32, (n, N))
+                ref = alone(hosts[k], storages[k], pts)[0]
+                np.testing.assert_array_equal(v, ref)
+        except Exception as exc:      # surfaced below
+            errors.append(exc)
+
+    th = [threading.Thread(target=worker, args=(s,)) for s in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors[0]
+    assert srv.stats()["completed"] == 100
+    srv.close()
