@@ -32,6 +32,7 @@ import einspline_amd as ea  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "orbital-vgh evals/sec + %HBM roofline, FP32 vs BF16 table; serving p99 latency"
+ORDER_NAMES = ("given", "locality")
 
 
 def log(msg):
@@ -144,10 +145,12 @@ def headline(args):
     if args.lane_bytes or args.block:
         tab.set_tile(args.lane_bytes, args.block)
         tab.set_store_policy(args.store_policy)
-        tile = (args.lane_bytes, args.block, args.store_policy)
+        tab.set_point_order(max(args.point_order, 0))
     else:
-        lb, blk, nt, _ = tab.autotune("vgh", P)
-        tile = (lb, blk, nt)
+        tab.autotune("vgh", P)
+        if args.point_order >= 0:
+            tab.set_point_order(args.point_order)
+    tile = tab.tuning()            # (lane bytes, block, store policy, point order)
     rng = np.random.default_rng(99)
     pts = rng.random((P, 3))
     pos = ea.DeviceBuffer.from_array(pts)
@@ -178,12 +181,15 @@ def headline(args):
                    "grid": [M, M, M], "orbitals": N, "points": P, "mode": "vgh",
                    "table": "f32", "hessian_layout": "packed6" if packed else "full9",
                    "table_bytes": int(tab.bytes), "tile_lane_bytes": tile[0],
-                   "tile_block": tile[1], "store_policy": tile[2]},
+                   "tile_block": tile[1], "store_policy": tile[2],
+                   "point_order": ORDER_NAMES[tile[3]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "bytes_per_eval": bpe, "kernel_ms": kern,
-                     "note": "table (135.8 MB) fits the 256 MB Infinity Cache; see traffic"},
+                     "note": "table (135.8 MB) fits the 256 MB Infinity Cache; see traffic"
+                             + ("; kernel_ms is the whole call: cell keys + radix sort + eval3d"
+                                if tile[3] == ea.ORDER_LOCALITY else "")},
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, host cores)")
@@ -200,6 +206,16 @@ def headline(args):
         res.setdefault("extra", {})["single_point_vgh_call_us"] = {
             "median": float(np.median(t) * 1e6), "p99": float(np.percentile(t, 99) * 1e6),
             "note": "eval_multi_UBspline_3d_s_vgh through ctypes: H2D position, launch, D2H of 256 x 13 floats"}
+    # the other traversal order at the same tile, for the record
+    if not args.no_extra:
+        other = 1 - tile[3]
+        tab.set_point_order(other)
+        w3, k3 = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 2, packed)
+        tab.set_point_order(tile[3])
+        res.setdefault("extra", {})["headline_" + ORDER_NAMES[other] + "_order"] = {
+            "evals_per_s": evals / (w3 * 1e-3), "kernel_ms": k3, "ms_per_step": w3,
+            "achieved_GBs": evals * bpe / (k3 * 1e-3) / 1e9}
+        log(f"[bench] FP32 vgh, {ORDER_NAMES[other]} order: {w3:.4f} ms/step wall")
     # layout variant for the record
     if not packed and not args.no_extra:
         outs_p = tab.alloc_outputs("vgh", P, True)
@@ -231,10 +247,10 @@ def precision(args, res):
         tab = ea.DeviceTable(host, storage=st)
         if name != "f32" and args.lane_bytes_16:
             tab.set_tile(args.lane_bytes_16, 0)
-            tile = (args.lane_bytes_16, 256, 0)
+            tile = tab.tuning()
         else:
-            lb, blk, nt, _ = tab.autotune("vgh", 32768)
-            tile = (lb, blk, nt)
+            tab.autotune("vgh", P)
+            tile = tab.tuning()
         outs = tab.alloc_outputs("vgh", P, args.packed)
         wall, kern = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 3, args.packed)
         # errors on the sample
@@ -356,7 +372,8 @@ def other_configs(args, res):
     pos = ea.DeviceBuffer.from_array(pts)
     for name, host, cb in (("z_vgl_64c_n128_p65536", hz, 16), ("c_vgl_64c_n128_p65536", hc, 8)):
         tab = ea.DeviceTable(host)
-        lb, blk, nt, _ = tab.autotune("vgl", 32768)
+        tab.autotune("vgl", 65536)
+        tile = tab.tuning()
         outs = tab.alloc_outputs("vgl", 65536)
         w, k = time_batches(tab, "vgl", pos, 65536, outs, max(5, args.steps // 2), 3)
         bpe = 64 * cb + 5 * cb
@@ -364,7 +381,7 @@ def other_configs(args, res):
         rows[name] = {"kernel_ms": k, "evals_per_s": ev / (w * 1e-3), "bytes_per_eval": bpe,
                       "achieved_GBs": ev * bpe / (k * 1e-3) / 1e9,
                       "frac_hbm": ev * bpe / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                      "table_bytes": int(tab.bytes), "tile": [lb, blk, nt]}
+                      "table_bytes": int(tab.bytes), "tile": list(tile)}
         tab.destroy()
     hd = ea.HostSpline("d", (32, 32, 32), (0, 0, 0), (1, 1, 1), 64)
     hd.set_block(0, rng.uniform(-1, 1, (64, 32, 32, 32)))
@@ -401,6 +418,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--store-policy", type=int, default=0,
                     help="with --lane-bytes/--block: 0 staged+streaming, 1 staged, 2 direct")
+    ap.add_argument("--point-order", type=int, default=-1,
+                    help="0 given, 1 locality (default: the autotuner's choice)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the spline evaluator is a single-GPU kernel "

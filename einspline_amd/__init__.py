@@ -33,6 +33,7 @@ SINGLE_REAL, DOUBLE_REAL, SINGLE_COMPLEX, DOUBLE_COMPLEX = range(4)
 MULTI_U3D = 8
 STORE_F32, STORE_F64, STORE_BF16, STORE_F16 = range(4)
 MODES = ("v", "vg", "vgl", "vgh")
+ORDER_GIVEN, ORDER_LOCALITY = 0, 1
 
 E_OK, E_INVAL, E_NOMEM, E_HIP, E_RANGE, E_BUDGET, E_BUSY = 0, -1, -2, -3, -4, -5, -6
 
@@ -148,6 +149,9 @@ _proto("einspline_set_tile", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i)
 _proto("einspline_autotune", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i, C.POINTER(_i),
        C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_float))
 _proto("einspline_set_store_policy", _i, C.POINTER(multi_UBspline_3d_hip), _i)
+_proto("einspline_set_point_order", _i, C.POINTER(multi_UBspline_3d_hip), _i)
+_proto("einspline_get_tuning", _i, C.POINTER(multi_UBspline_3d_hip), C.POINTER(_i), C.POINTER(_i),
+       C.POINTER(_i), C.POINTER(_i))
 _proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp)
 _proto("einspline_eval_vg_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp)
 _proto("einspline_eval_vgl_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp)
@@ -422,6 +426,17 @@ class DeviceTable:
     def set_store_policy(self, policy=0):
         """0 staged + streaming (default), 1 staged + ordinary, 2 direct."""
         check(lib.einspline_set_store_policy(self.ptr, int(policy)), "set_store_policy")
+
+    def set_point_order(self, order=ORDER_GIVEN):
+        """ORDER_GIVEN evaluates points as passed; ORDER_LOCALITY sorts each
+        batch of >= 1024 points by cell first (same results, better reuse)."""
+        check(lib.einspline_set_point_order(self.ptr, int(order)), "set_point_order")
+
+    def tuning(self):
+        """(lane_bytes, block, store_policy, point_order) in effect."""
+        v = [_i() for _ in range(4)]
+        check(lib.einspline_get_tuning(self.ptr, *[C.byref(x) for x in v]), "get_tuning")
+        return tuple(x.value for x in v)
 
     def autotune(self, mode="vgh", sample_points=8192):
         """Pick the fastest tile and store policy for `mode` on this table

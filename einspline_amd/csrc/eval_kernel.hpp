@@ -63,6 +63,8 @@ struct Params {
   const void *sign;          // optional per-point factor applied to every output (or null)
   long row_stride;           // MODE_QMC: distance between gradient/Laplacian rows, in reals
   int store;                 // result store policy: STORE_STAGED_NT, STORE_STAGED, STORE_DIRECT
+  const int *perm;           // optional traversal order: slot s evaluates point perm[s]
+  int xcd_runs;              // 1: each XCD takes one contiguous run of blocks
 };
 
 // Result store policies.  STAGED: gradient/Hessian blocks of a wavefront go
@@ -646,11 +648,23 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
 template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS>
 __global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) eval3d(const Params P) {
   // npoints * chunks < 2^31 is checked on the host, so 32-bit math suffices
-  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned b = blockIdx.x;
+  if (P.xcd_runs) {
+    // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on
+    // XCD b % 8).  Give XCD x the x-th contiguous run of logical blocks, so
+    // consecutive slots of a sorted traversal share that XCD's L2.
+    const unsigned nb = gridDim.x, q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
+    b = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+  }
+  const unsigned gid = b * blockDim.x + threadIdx.x;
   int p = (int)(gid / (unsigned)P.chunks);
   if (UNIF) p = uniform(p);
   const int ch = (int)(gid - (unsigned)p * (unsigned)P.chunks);
   if (p >= P.npoints) return;
+  if (P.perm) {
+    p = P.perm[p];
+    if (UNIF) p = uniform(p);
+  }
   eval_chunk<S, LB, MODE, CPLX, UNIF, PosT, PTRS>(P, p, ch);
 }
 

@@ -119,7 +119,13 @@ struct TableExt {
   int lane_bytes;              // tile: bytes per lane per row (0 = default)
   int block;                   // tile: work-items per block (0 = default)
   int store_policy;            // esk::STORE_* for result stores
+  int point_order;             // EINSPLINE_ORDER_*
+  EskOrderWs *order_ws;        // sort workspace of the locality order (lazily made)
 };
+
+// Batches at least this large use the locality order when the table asks
+// for it; smaller ones gain nothing from sorting.
+const int kOrderMinPoints = 1024;
 
 // host memcpy split over a few threads (large chunks are memory-bound)
 void par_memcpy(void *dst, const void *src, size_t n) {
@@ -296,6 +302,7 @@ extern "C" multi_UBspline_3d_z_hip *create_multi_UBspline_3d_z_hip(multi_UBsplin
 
 extern "C" void destroy_multi_UBspline_3d_hip(multi_UBspline_3d_hip *t) {
   if (!t) return;
+  esk_order_free(((TableExt *)t)->order_ws);
   if (t->coefs) (void)hipFree(t->coefs);
   free((TableExt *)t);
 }
@@ -325,19 +332,39 @@ extern "C" int einspline_set_store_policy(multi_UBspline_3d_hip *t, int policy) 
   return EINSPLINE_OK;
 }
 
+extern "C" int einspline_set_point_order(multi_UBspline_3d_hip *t, int order) {
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
+  if (order != EINSPLINE_ORDER_GIVEN && order != EINSPLINE_ORDER_LOCALITY)
+    return fail(EINSPLINE_EINVAL, "point order must be EINSPLINE_ORDER_GIVEN or _LOCALITY");
+  ((TableExt *)t)->point_order = order;
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_get_tuning(const multi_UBspline_3d_hip *t, int *lane_bytes, int *block,
+                                    int *store_policy, int *point_order) {
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
+  const TableExt *x = (const TableExt *)t;
+  if (lane_bytes) *lane_bytes = x->lane_bytes ? x->lane_bytes : 16;
+  if (block) *block = x->block ? x->block : 256;
+  if (store_policy) *store_policy = x->store_policy;
+  if (point_order) *point_order = x->point_order;
+  return EINSPLINE_OK;
+}
+
 // ---------------------------------------------------------------------------
 // autotuner
 
 namespace {
 std::mutex g_tune_mu;
 struct TuneKey {
-  int storage, tcode, mode, row_bytes;
+  int storage, tcode, mode, row_bytes, sample;
   bool operator<(const TuneKey &o) const {
-    return std::tie(storage, tcode, mode, row_bytes) < std::tie(o.storage, o.tcode, o.mode, o.row_bytes);
+    return std::tie(storage, tcode, mode, row_bytes, sample) <
+           std::tie(o.storage, o.tcode, o.mode, o.row_bytes, o.sample);
   }
 };
 struct TuneVal {
-  int lane_bytes, block, store;
+  int lane_bytes, block, store, order;
 };
 std::map<TuneKey, TuneVal> g_tune;
 bool g_tune_loaded = false;
@@ -351,8 +378,8 @@ void tune_load() {
   if (!fp) return;
   TuneKey k;
   TuneVal v;
-  while (fscanf(fp, "%d %d %d %d %d %d %d", &k.storage, &k.tcode, &k.mode, &k.row_bytes,
-                &v.lane_bytes, &v.block, &v.store) == 7)
+  while (fscanf(fp, "%d %d %d %d %d %d %d %d %d", &k.storage, &k.tcode, &k.mode, &k.row_bytes,
+                &k.sample, &v.lane_bytes, &v.block, &v.store, &v.order) == 9)
     g_tune[k] = v;
   fclose(fp);
 }
@@ -363,8 +390,9 @@ void tune_save() {
   FILE *fp = fopen(f, "w");
   if (!fp) return;
   for (auto &kv : g_tune)
-    fprintf(fp, "%d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
-            kv.first.row_bytes, kv.second.lane_bytes, kv.second.block, kv.second.store);
+    fprintf(fp, "%d %d %d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
+            kv.first.row_bytes, kv.first.sample, kv.second.lane_bytes, kv.second.block,
+            kv.second.store, kv.second.order);
   fclose(fp);
 }
 }  // namespace
@@ -379,7 +407,7 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   if (!t || mode < 0 || mode > 3) return fail(EINSPLINE_EINVAL, "bad autotune arguments");
   if (sample <= 0) sample = 8192;
   const int eb = (int)storage_bytes(t->storage);
-  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb};
+  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb, sample};
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     tune_load();
@@ -387,6 +415,7 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
     if (it != g_tune.end()) {
       einspline_set_tile(t, it->second.lane_bytes, it->second.block);
       einspline_set_store_policy(t, it->second.store);
+      einspline_set_point_order(t, it->second.order);
       if (lb_out) *lb_out = it->second.lane_bytes;
       if (blk_out) *blk_out = it->second.block;
       if (nt_out) *nt_out = it->second.store;
@@ -421,12 +450,15 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
     rc = hip_fail(e, "autotune setup");
   TableExt *ext = (TableExt *)t;
   const int old_lb = ext->lane_bytes, old_blk = ext->block, old_store = ext->store_policy;
-  int best_lb = 16, best_blk = 256, best_nt = esk::STORE_STAGED_NT;
+  const int old_order = ext->point_order;
+  int best_lb = 16, best_blk = 256, best_nt = esk::STORE_STAGED_NT, best_order = EINSPLINE_ORDER_GIVEN;
   float best = -1.0f, default_ms = -1.0f;
   const int lbs_f64[] = {16, 32}, lbs_other[] = {8, 16, 32};
   const int *lbs = t->storage == EINSPLINE_STORE_F64 ? lbs_f64 : lbs_other;
   const int nlb = t->storage == EINSPLINE_STORE_F64 ? 2 : 3;
   const int blks[] = {64, 128, 256};
+  const int norders = sample >= kOrderMinPoints ? 2 : 1;
+  for (int order = 0; order < norders; ++order)
   for (int nt = 0; nt <= 2; ++nt)
   for (int a = 0; rc == EINSPLINE_OK && a < nlb; ++a) {
     if ((t->row_reals * eb) % lbs[a]) continue;
@@ -434,6 +466,7 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
       ext->lane_bytes = lbs[a];
       ext->block = blks[b];
       ext->store_policy = nt;
+      ext->point_order = order;
       auto launch = [&]() {
         return run_strided(t, mode, (const double *)dp, sample, dv, dg, dh, 0, st);
       };
@@ -451,20 +484,24 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
         if (ms < 0.0f || t_ms < ms) ms = t_ms;
       }
       if (rc != EINSPLINE_OK) break;
-      if (lbs[a] == 16 && blks[b] == 256 && nt == esk::STORE_STAGED_NT) default_ms = ms;
-      if (best < 0.0f || ms < best) { best = ms; best_lb = lbs[a]; best_blk = blks[b]; best_nt = nt; }
+      if (lbs[a] == 16 && blks[b] == 256 && nt == esk::STORE_STAGED_NT && order == 0) default_ms = ms;
+      if (best < 0.0f || ms < best) {
+        best = ms; best_lb = lbs[a]; best_blk = blks[b]; best_nt = nt; best_order = order;
+      }
     }
   }
   ext->lane_bytes = old_lb;
   ext->block = old_blk;
   ext->store_policy = old_store;
-  // keep the default tile (16 B x 256, streaming stores) unless the winner is
-  // at least 3% faster
+  ext->point_order = old_order;
+  // keep the default (16 B x 256, streaming stores, given order) unless the
+  // winner is at least 3% faster
   if (default_ms > 0.0f && best > 0.97f * default_ms) {
     best = default_ms;
     best_lb = 16;
     best_blk = 256;
     best_nt = esk::STORE_STAGED_NT;
+    best_order = EINSPLINE_ORDER_GIVEN;
   }
   if (dp) (void)hipFree(dp);
   if (dv) (void)hipFree(dv);
@@ -476,9 +513,10 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   if (rc != EINSPLINE_OK) return rc;
   einspline_set_tile(t, best_lb, best_blk);
   einspline_set_store_policy(t, best_nt);
+  einspline_set_point_order(t, best_order);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = {best_lb, best_blk, best_nt};
+    g_tune[key] = {best_lb, best_blk, best_nt, best_order};
     tune_save();
   }
   if (lb_out) *lb_out = best_lb;
@@ -536,6 +574,23 @@ static hipError_t dispatch(const multi_UBspline_3d_hip *t, const Launch &L, cons
   return hipErrorInvalidValue;
 }
 
+// Sort the batch for the locality order when the table asks for it.  Not
+// under stream capture: the sort workspace is shared by the table's calls.
+static int order_begin(const multi_UBspline_3d_hip *t, Params &P, const void *pos, bool pos_f32,
+                       int num, hipStream_t s, EskOrderWs **ws) {
+  *ws = nullptr;
+  TableExt *x = (TableExt *)t;
+  if (x->point_order != EINSPLINE_ORDER_LOCALITY || num < kOrderMinPoints) return EINSPLINE_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return EINSPLINE_OK;
+  int rc = esk_order_begin(&x->order_ws, P, pos, pos_f32, num, s, &P.perm);
+  if (rc) return rc;
+  P.xcd_runs = 1;
+  *ws = x->order_ws;
+  return EINSPLINE_OK;
+}
+
 static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
                        void *v, void *g, void *h, int packed, void *stream) {
   Launch L;
@@ -553,7 +608,10 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
   P.ld_g = 3l * P.nreal;
   P.ld_h = mode == esk::MODE_VGL ? (long)P.nreal : (packed ? 6l : 9l) * P.nreal;
   P.hpacked = packed ? 1 : 0;
+  EskOrderWs *ws;
+  if ((rc = order_begin(t, P, pos, false, num, (hipStream_t)stream, &ws))) return rc;
   hipError_t e = dispatch(t, L, P, (hipStream_t)stream);
+  if (ws) esk_order_end(ws, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return EINSPLINE_OK;
 }
@@ -605,7 +663,10 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
   P.linv = linv;
   P.sign = sign;
   P.row_stride = (long)row_stride * (L.cplx ? 2 : 1);
+  EskOrderWs *ws;
+  if ((rc = order_begin(t, P, pos, pos_f32, num, nullptr, &ws))) return rc;
   hipError_t e = dispatch(t, L, P, nullptr);
+  if (ws) esk_order_end(ws, nullptr);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");

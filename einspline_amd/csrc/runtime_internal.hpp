@@ -11,3 +11,13 @@ int esk_hip_fail(hipError_t e, const char *what);
 int esk_prepare(const multi_UBspline_3d_hip *t, int num, esk::Launch &L, esk::Params &P);
 // Wrap device memory holding a table laid out as `meta` (takes ownership).
 multi_UBspline_3d_hip *esk_table_wrap(const multi_UBspline_3d_hip *meta, void *dcoefs);
+
+// Locality traversal (order.hip).  esk_order_begin sorts the batch's points
+// by cell on stream `s` into the table's workspace (*slot, created on first
+// use) and returns the permutation; the workspace stays locked until
+// esk_order_end records the stream's last launch that reads it.
+struct EskOrderWs;
+void esk_order_free(EskOrderWs *w);
+int esk_order_begin(EskOrderWs **slot, const esk::Params &P, const void *pos, bool pos_f32, int num,
+                    hipStream_t s, const int **perm);
+void esk_order_end(EskOrderWs *w, hipStream_t s);

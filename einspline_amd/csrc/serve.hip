@@ -661,7 +661,6 @@ extern "C" int einspline_server_run_load(einspline_server *s, const einspline_lo
   }
   const auto t_start = clk::now();
   double t_next = 0.0;
-  size_t n = 0;
   for (;;) {
     if (ld->rate_hz > 0) t_next += gap(rng);
     if (t_next >= ld->duration_s) break;
@@ -690,7 +689,6 @@ extern "C" int einspline_server_run_load(einspline_server *s, const einspline_lo
     int rc = submit_impl(s, ld->table_ids[k], pos.data(), np, ld->vals_d[k], ld->grads_d[k],
                          ld->hess_d[k], nullptr);
     if (rc) return rc;
-    ++n;
   }
   int rc = einspline_server_drain(s);
   const double wall = std::chrono::duration<double>(clk::now() - t_start).count();

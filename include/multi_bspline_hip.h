@@ -181,12 +181,34 @@ int einspline_set_tile(multi_UBspline_3d_hip *spline, int lane_bytes, int block_
  *   2 each lane stores its own results with ordinary stores. */
 int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
 
+/* Point traversal order (no reference counterpart; the reference evaluates
+ * points in the order given).
+ *   EINSPLINE_ORDER_GIVEN     one point after another as passed (default);
+ *   EINSPLINE_ORDER_LOCALITY  batches of 1024 points or more are first sorted
+ *     by the Morton code of their cell (a key kernel and a radix sort on the
+ *     call's stream), and each XCD then walks one contiguous run of the sorted
+ *     points, so that points sharing coefficient rows run together and the
+ *     rows are served from L2 and the Infinity Cache instead of HBM.
+ * Results are identical in both orders: each point's outputs go to its own
+ * index and are computed by the same instructions.  The sort uses a workspace
+ * owned by the table (allocated on first use, grown with hipMalloc when a
+ * larger batch arrives); calls on the same table from different streams are
+ * ordered one after another through it.  Under stream capture the given order
+ * is used. */
+enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
+int einspline_set_point_order(multi_UBspline_3d_hip *spline, int order);
+
+/* The table's current tile, store policy and point order (defaults filled in). */
+int einspline_get_tuning(const multi_UBspline_3d_hip *spline, int *lane_bytes, int *block_threads,
+                         int *store_policy, int *point_order);
+
 /* Autotuner (no reference counterpart): times every supported tile shape
- * (lane_bytes x block_threads x store policy) of `mode` (0 v, 1 vg, 2 vgl, 3 vgh) on
- * `sample_points` uniform random points of the table's box, applies the
- * fastest with einspline_set_tile / einspline_set_store_policy and
- * returns it.  Results are cached per
- * (storage, type, mode, row bytes) for the process, and in the text file
+ * (lane_bytes x block_threads x store policy x point order) of `mode` (0 v, 1 vg, 2 vgl,
+ * 3 vgh) on `sample_points` uniform random points of the table's box, applies the
+ * fastest with einspline_set_tile / einspline_set_store_policy /
+ * einspline_set_point_order and returns the tile and policy (the order is read
+ * back with einspline_get_tuning).  Results are cached per
+ * (storage, type, mode, row bytes, sample size) for the process, and in the text file
  * named by $EINSPLINE_TUNE_CACHE when set.  Synchronises the device. */
 int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_points,
                        int *lane_bytes, int *block_threads, int *store_policy,

@@ -24,7 +24,7 @@ if not os.path.exists(LIB_PATH):
 lib = C.CDLL(LIB_PATH)
 
 OR_PERIODIC, OR_DERIV1, OR_DERIV2, OR_FLAT, OR_NATURAL, OR_ANTIPERIODIC = range(6)
-MODE = {"v": 0, "vg": 1, "vgl": 2, "vgh": 3}
+MODE = {"v": 0, "vg": 1, "vgl": 2, "vgh": 3, "vghgh": 4}
 
 
 class otable(C.Structure):
@@ -34,12 +34,12 @@ class otable(C.Structure):
                 ("lcode", C.c_int * 3), ("rcode", C.c_int * 3),
                 ("lval", C.c_double * 6), ("rval", C.c_double * 6),
                 ("N", C.c_int), ("ncomp", C.c_int),
-                ("xs", C.c_long), ("ys", C.c_long), ("zs", C.c_long)]
+                ("xs", C.c_long), ("ys", C.c_long), ("zs", C.c_long), ("dim", C.c_int)]
 
 
 _vp, _i, _d = C.c_void_p, C.c_int, C.c_double
 lib.oracle_table_init.restype = _i
-lib.oracle_table_init.argtypes = [C.POINTER(otable), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i]
+lib.oracle_table_init.argtypes = [C.POINTER(otable), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]
 for k in "sdcz":
     f = getattr(lib, f"oracle_set_{k}")
     f.restype = None
@@ -47,11 +47,13 @@ for k in "sdcz":
 for k in "sd":
     f = getattr(lib, f"oracle_eval_{k}")
     f.restype = _i
-    f.argtypes = [C.POINTER(otable), _vp, _i, _d, _d, _d, _vp, _vp, _vp]
+    f.argtypes = [C.POINTER(otable), _vp, _i, _d, _d, _d, _vp, _vp, _vp, _vp]
 lib.oracle_eval_batch.restype = _i
-lib.oracle_eval_batch.argtypes = [C.POINTER(otable), _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _i]
+lib.oracle_eval_batch.argtypes = [C.POINTER(otable), _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _i]
 lib.oracle_basis_d.restype = None
 lib.oracle_basis_d.argtypes = [_d, _vp, _vp, _vp]
+lib.oracle_basis3_d.restype = None
+lib.oracle_basis3_d.argtypes = [_vp]
 lib.oracle_max_threads.restype = _i
 
 
@@ -61,18 +63,27 @@ def basis(t: float):
     return a, da, d2a
 
 
+def basis3():
+    d3a = np.zeros(4)
+    lib.oracle_basis3_d(d3a.ctypes.data)
+    return d3a
+
+
 def max_threads() -> int:
     return lib.oracle_max_threads()
 
 
 class Table:
-    """Oracle spline table of kind s/d/c/z (numpy-owned coefficients)."""
+    """Oracle spline table of kind s/d/c/z (numpy-owned coefficients), in
+    len(M) = 1, 2 or 3 dimensions."""
 
     def __init__(self, kind, M, start, end, N, bc=None):
         self.kind = kind
+        self.dim = dim = len(M)
+        M, start, end = (tuple(v) + (1,) * (3 - dim) for v in (M, start, end))
         self.ncomp = 2 if kind in "cz" else 1
         self.real = np.float32 if kind in "sc" else np.float64
-        bc = bc or [(OR_PERIODIC, OR_PERIODIC)] * 3
+        bc = list(bc or [(OR_PERIODIC, OR_PERIODIC)] * dim) + [(OR_PERIODIC, OR_PERIODIC)] * (3 - dim)
         lc = np.array([b[0] for b in bc], np.int32)
         rc = np.array([b[1] for b in bc], np.int32)
         lv = np.zeros(6)
@@ -88,16 +99,16 @@ class Table:
         e_a = np.array(end, float)
         rc_ = lib.oracle_table_init(C.byref(self.t), m_a.ctypes.data, s_a.ctypes.data,
                                     e_a.ctypes.data, lc.ctypes.data, rc.ctypes.data,
-                                    lv.ctypes.data, rv.ctypes.data, int(N), self.ncomp)
+                                    lv.ctypes.data, rv.ctypes.data, int(N), self.ncomp, dim)
         assert rc_ == 0, "invalid oracle table"
         self.N = int(N)
-        self.M = tuple(M)
-        self.shape = tuple(self.t.Nn)
+        self.M = tuple(M[:dim])
+        self.shape = tuple(self.t.Nn[:dim])
         self.coefs = np.zeros(self.shape + (self.N * self.ncomp,), self.real)
 
     @property
     def delta_inv(self):
-        return np.array(self.t.delta_inv[:])
+        return np.array(self.t.delta_inv[:self.dim])
 
     def set(self, n, data):
         data = np.asarray(data)
@@ -111,25 +122,29 @@ class Table:
                                                  data.ctypes.data)
 
     def eval_batch(self, mode, pos, nthreads=0):
-        """pos: (P, 3) float64.  Returns (vals, grads, lapl/hess) as arrays of
-        shape (P, N), (P, N, 3), (P, N) or (P, N, 3, 3); complex for c/z."""
-        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        """pos: (P, dim) float64.  Returns (vals, grads, lapl/hess[, gradhess])
+        shaped (P, N), (P, N, dim), (P, N) or (P, N, dim, dim) (1-D vgh: (P, N)),
+        (P, N, 3, 3, 3); complex for c/z."""
+        dim = self.dim
+        pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, dim))
         P = pos.shape[0]
         NR = self.N * self.ncomp
         m = MODE[mode]
+        assert mode != "vghgh" or dim == 3
         v = np.zeros((P, NR), self.real)
-        g = np.zeros((P, 3 * NR), self.real) if m >= 1 else None
-        nh = 9 if mode == "vgh" else 1 if mode == "vgl" else 0
+        g = np.zeros((P, dim * NR), self.real) if m >= 1 else None
+        nh = {"vgl": 1, "vgh": 1 if dim == 1 else dim * dim, "vghgh": 9}.get(mode, 0)
         h = np.zeros((P, nh * NR), self.real) if nh else None
+        gh = np.zeros((P, 27 * NR), self.real) if mode == "vghgh" else None
+        ptr = lambda a: a.ctypes.data if a is not None else None
         bad = lib.oracle_eval_batch(C.byref(self.t), 1 if self.real == np.float64 else 0,
                                     self.coefs.ctypes.data, m, pos.ctypes.data, P, v.ctypes.data,
-                                    g.ctypes.data if g is not None else None,
-                                    h.ctypes.data if h is not None else None, int(nthreads))
+                                    ptr(g), ptr(h), ptr(gh), int(nthreads))
         self.bad = bad
-        return shape_results(mode, self.N, self.ncomp, v, g, h)
+        return shape_results(mode, self.N, self.ncomp, v, g, h, gh, dim)
 
 
-def shape_results(mode, N, ncomp, v, g, h):
+def shape_results(mode, N, ncomp, v, g, h, gh=None, dim=3):
     """Reshape flat per-point result blocks (CPU layout) to (P, N, ...)."""
     P = v.shape[0]
 
@@ -140,10 +155,12 @@ def shape_results(mode, N, ncomp, v, g, h):
 
     out = [cx(v).reshape(P, N)]
     if g is not None:
-        out.append(cx(g).reshape(P, N, 3))
+        out.append(cx(g).reshape(P, N, dim))
     if h is not None:
-        if mode == "vgl":
+        if mode == "vgl" or dim == 1:
             out.append(cx(h).reshape(P, N))
         else:
-            out.append(cx(h).reshape(P, N, 3, 3))
+            out.append(cx(h).reshape(P, N, dim, dim))
+    if gh is not None:
+        out.append(cx(gh).reshape(P, N, 3, 3, 3))
     return tuple(out)

@@ -9,8 +9,9 @@ dropped halo copy, a wrong delta_inv power) gives a different answer.
     B3(s) = (2-|s|)^3/6 for 1 <= |s| < 2,  (4 - 6 s^2 + 3|s|^3)/6 for |s| < 1,
   and a spline with stored coefficients c_j is f(u) = sum_j c_j B3(u - j + 1),
   u = (x - start) * delta_inv (the stored index j is the data index + 1).
-  The 3-D value is the full tensor-product sum over the whole table, with no
-  stencil selection at all.
+  The value is the full tensor-product sum over the whole table (1, 2 or 3
+  axes), with no stencil selection at all; derivatives up to the third come
+  from the piecewise-polynomial derivatives of B3.
 * Interpolation maps: dense per-axis linear maps from data to coefficients,
   built from the interpolation conditions and solved with numpy.linalg.
 * FFT periodic solve: eigenvalues (4 + 2 cos(2 pi k / M)) / 6 of the
@@ -41,43 +42,55 @@ def d2b3(s):
     return np.where(a < 1, (-12 + 18 * a) / 6, np.where(a < 2, (2 - a), 0.0))
 
 
+def d3b3(s):
+    """Third derivative: piecewise constant, 3 sign(s) inside |s| < 1 and
+    -sign(s) on 1 <= |s| < 2 (undefined at the knots)."""
+    s = np.asarray(s, float)
+    a = np.abs(s)
+    return np.sign(s) * np.where(a < 1, 3.0, np.where(a < 2, -1.0, 0.0))
+
+
 def axis_weights(x, start, dinv, n):
     """Weights of all n stored coefficients along one axis at coordinate x:
-    value, first and second derivative (derivatives in physical units)."""
+    value and first, second, third derivative (derivatives in physical units)."""
     u = (x - start) * dinv
     j = np.arange(n)
     s = u - j + 1
-    return b3(s), db3(s) * dinv, d2b3(s) * dinv**2
+    return b3(s), db3(s) * dinv, d2b3(s) * dinv**2, d3b3(s) * dinv**3
 
 
 def eval_dense(coefs, start, dinv, pts):
-    """Full tensor-product evaluation.  coefs: (Nx, Ny, Nz, R) reals or complex.
-    Returns v (P,R), g (P,R,3), H (P,R,3,3)."""
-    Nx, Ny, Nz = coefs.shape[:3]
-    P = len(pts)
-    R = coefs.shape[3]
+    """Full tensor-product evaluation in dim = len(start) dimensions.
+    coefs: (N_0, .., N_{dim-1}, R) reals or complex.  Returns v (P,R),
+    g (P,R,dim), H (P,R,dim,dim) and T (P,R,dim,dim,dim), T the third
+    derivatives."""
+    dim = len(start)
+    shape = coefs.shape[:dim]
+    pts = np.asarray(pts, float).reshape(-1, dim)
+    P, R = len(pts), coefs.shape[dim]
     v = np.zeros((P, R), coefs.dtype)
-    g = np.zeros((P, R, 3), coefs.dtype)
-    H = np.zeros((P, R, 3, 3), coefs.dtype)
-    for p, (x, y, z) in enumerate(pts):
-        wx = axis_weights(x, start[0], dinv[0], Nx)
-        wy = axis_weights(y, start[1], dinv[1], Ny)
-        wz = axis_weights(z, start[2], dinv[2], Nz)
+    g = np.zeros((P, R, dim), coefs.dtype)
+    H = np.zeros((P, R, dim, dim), coefs.dtype)
+    T = np.zeros((P, R, dim, dim, dim), coefs.dtype)
+    for p in range(P):
+        w = [axis_weights(pts[p, a], start[a], dinv[a], shape[a]) for a in range(dim)]
 
-        def con(ox, oy, oz):
-            return np.einsum("i,j,k,ijkr->r", wx[ox], wy[oy], wz[oz], coefs)
+        def con(*axes):
+            """Contract with d/dx_{axes[0]} d/dx_{axes[1]} ... (order per axis)."""
+            order = [list(axes).count(a) for a in range(dim)]
+            out = coefs
+            for a in range(dim):
+                out = np.tensordot(w[a][order[a]], out, axes=(0, 0))
+            return out
 
-        v[p] = con(0, 0, 0)
-        g[p, :, 0] = con(1, 0, 0)
-        g[p, :, 1] = con(0, 1, 0)
-        g[p, :, 2] = con(0, 0, 1)
-        H[p, :, 0, 0] = con(2, 0, 0)
-        H[p, :, 1, 1] = con(0, 2, 0)
-        H[p, :, 2, 2] = con(0, 0, 2)
-        H[p, :, 0, 1] = H[p, :, 1, 0] = con(1, 1, 0)
-        H[p, :, 0, 2] = H[p, :, 2, 0] = con(1, 0, 1)
-        H[p, :, 1, 2] = H[p, :, 2, 1] = con(0, 1, 1)
-    return v, g, H
+        v[p] = con()
+        for a in range(dim):
+            g[p, :, a] = con(a)
+            for b2 in range(dim):
+                H[p, :, a, b2] = con(a, b2)
+                for c in range(dim):
+                    T[p, :, a, b2, c] = con(a, b2, c)
+    return v, g, H, T
 
 
 def interp_map(M, lcode, rcode, dinv):
@@ -116,21 +129,23 @@ def interp_map(M, lcode, rcode, dinv):
 
 
 def solve_dense(data, bcs, dinv, lr_vals=None):
-    """3-D coefficients for data (Mx, My, Mz[, R]) by applying the dense
-    per-axis maps along x, then y, then z.  lr_vals[a] = (lval, rval)."""
+    """Coefficients for data (M_0, .., M_{dim-1}[, R]), dim = len(bcs), by
+    applying the dense per-axis maps along x, then y, then z.
+    lr_vals[a] = (lval, rval)."""
+    dim = len(bcs)
     data = np.asarray(data)
-    squeeze = data.ndim == 3
+    squeeze = data.ndim == dim
     if squeeze:
         data = data[..., None]
     out = data
-    for a in range(3):
+    for a in range(dim):
         M = data.shape[a]
         P = interp_map(M, bcs[a][0], bcs[a][1], dinv[a])
         lv, rv = (lr_vals[a] if lr_vals is not None else (0.0, 0.0))
         ext_shape = list(out.shape)
         ext_shape[a] = M + 2
         ext = np.zeros(ext_shape, dtype=np.result_type(out.dtype, np.asarray(lv).dtype, float))
-        sl = [slice(None)] * 4
+        sl = [slice(None)] * (dim + 1)
         sl[a] = slice(0, M)
         ext[tuple(sl)] = out
         sl[a] = M
@@ -142,10 +157,15 @@ def solve_dense(data, bcs, dinv, lr_vals=None):
 
 
 def solve_fft_periodic(data):
-    """Periodic coefficients by FFT; data (Mx, My, Mz).  Returns (Mx+3, My+3, Mz+3)."""
+    """Periodic coefficients by FFT; data (M_0, .., M_{dim-1}).  Returns
+    (M_0+3, .., M_{dim-1}+3)."""
     d = np.asarray(data, float)
-    lam = [(4 + 2 * np.cos(2 * np.pi * np.arange(M) / M)) / 6 for M in d.shape]
-    den = lam[0][:, None, None] * lam[1][None, :, None] * lam[2][None, None, :]
+    den = np.ones(d.shape)
+    for a, M in enumerate(d.shape):
+        lam = (4 + 2 * np.cos(2 * np.pi * np.arange(M) / M)) / 6
+        sh = [1] * d.ndim
+        sh[a] = M
+        den = den * lam.reshape(sh)
     x = np.real(np.fft.ifftn(np.fft.fftn(d) / den))
     # stored index j holds x[(j - 1) mod M]
     idx = [(np.arange(M + 3) - 1) % M for M in d.shape]

@@ -55,14 +55,17 @@ static const float dA_f[16] = {0.0, -0.5, 1.0, -0.5, 0.0, 1.5, -2.0, 0.0,
 static const float d2A_f[16] = {0.0, 0.0, -1.0, 1.0, 0.0, 0.0, 3.0, -2.0,
                                 0.0, 0.0, -3.0, 1.0, 0.0, 0.0, 1.0, 0.0};
 
-/* ---- table geometry (src/multi_bspline_create.c:268-300) ---- */
+/* ---- table geometry (src/multi_bspline_create.c:268-300 for 3-D; the
+ * 1-D and 2-D creates, :767-820 and :840-901, use the same per-axis rules) */
 int oracle_table_init(otable *t, const int M[3], const double start[3], const double end[3],
                       const int lcode[3], const int rcode[3], const double lval[6],
-                      const double rval[6], int N, int ncomp) {
+                      const double rval[6], int N, int ncomp, int dim) {
   memset(t, 0, sizeof *t);
-  if (N < 1 || (ncomp != 1 && ncomp != 2)) return -1;
-  int Nn[3];
+  if (N < 1 || (ncomp != 1 && ncomp != 2) || dim < 1 || dim > 3) return -1;
+  int Nn[3] = {1, 1, 1};
   for (int a = 0; a < 3; ++a) {
+    t->M[a] = 1;
+    if (a >= dim) continue;          /* inactive axis: one row, never offset */
     if (M[a] < 2) return -1;
     t->M[a] = M[a];
     t->start[a] = start[a];
@@ -72,14 +75,15 @@ int oracle_table_init(otable *t, const int M[3], const double start[3], const do
     t->lval[a][0] = lval[2 * a]; t->lval[a][1] = lval[2 * a + 1];
     t->rval[a][0] = rval[2 * a]; t->rval[a][1] = rval[2 * a + 1];
     Nn[a] = (lcode[a] == OR_PERIODIC || lcode[a] == OR_ANTIPERIODIC) ? M[a] + 3 : M[a] + 2;
-    t->Nn[a] = Nn[a];
     t->delta[a] = (end[a] - start[a]) / (double)(Nn[a] - 3);
     t->delta_inv[a] = 1.0 / t->delta[a];
   }
+  for (int a = 0; a < 3; ++a) t->Nn[a] = Nn[a];
   t->N = N;
   t->ncomp = ncomp;
-  t->zs = N;
-  t->ys = (long)Nn[2] * N;
+  t->dim = dim;
+  t->zs = dim >= 3 ? N : 0;
+  t->ys = dim >= 2 ? (long)Nn[2] * N : 0;
   t->xs = (long)Nn[1] * Nn[2] * N;
   return 0;
 }
@@ -203,29 +207,33 @@ size_t oracle_table_elems(const otable *t) {
     deriv_##SFX(bands, coefs, M, cs);                                              \
     free(bands);                                                                   \
   }                                                                                \
-  /* 3-D set of orbital n: data[M0][M1][M2] (ncomp reals per element) */         \
-  static void set3d_##SFX(const otable *t, T *coefs, int n, const T *data) {        \
-    const int nc = t->ncomp;                                                       \
+  /* Set of orbital n: data[M0][M1][M2] (ncomp reals per element; axes past  \
+   * dim have M = 1).  Sweeps x, then y, then z, as the reference's 1-D, 2-D  \
+   * and 3-D sets do (multi_bspline_create.c:822-836, :903-935, :1020-1073). */ \
+  static void setnd_##SFX(const otable *t, T *coefs, int n, const T *data) {        \
+    const int nc = t->ncomp, dim = t->dim;                                         \
     const int My = t->M[1], Mz = t->M[2];                                          \
     const int Nx = t->Nn[0], Ny = t->Nn[1], Nz = t->Nn[2];                         \
-    const long zs = t->zs;                                                         \
+    const long es = (long)t->N;  /* elements per grid node */                     \
     T *c = coefs + (long)n * nc;                                                    \
     for (int comp = 0; comp < nc; ++comp) {                                        \
       for (int iy = 0; iy < My; ++iy)                                              \
         for (int iz = 0; iz < Mz; ++iz)                                            \
           line_##SFX(t, 0, comp, data + (long)nc * (iy * Mz + iz) + comp,          \
-                     (long)nc * My * Mz, c + (long)nc * (iy * Nz + iz) * zs + comp, \
-                     (long)nc * Ny * Nz * zs);                                     \
-      for (int ix = 0; ix < Nx; ++ix)                                              \
-        for (int iz = 0; iz < Mz; ++iz) {                                          \
-          T *p = c + (long)nc * ((long)ix * Ny * Nz + iz) * zs + comp;             \
-          line_##SFX(t, 1, comp, p, (long)nc * Nz * zs, p, (long)nc * Nz * zs);    \
-        }                                                                          \
-      for (int ix = 0; ix < Nx; ++ix)                                              \
-        for (int iy = 0; iy < Ny; ++iy) {                                          \
-          T *p = c + (long)nc * (((long)ix * Ny + iy) * Nz) * zs + comp;           \
-          line_##SFX(t, 2, comp, p, (long)nc * zs, p, (long)nc * zs);              \
-        }                                                                          \
+                     (long)nc * My * Mz, c + (long)nc * (iy * Nz + iz) * es + comp, \
+                     (long)nc * Ny * Nz * es);                                     \
+      if (dim >= 2)                                                                \
+        for (int ix = 0; ix < Nx; ++ix)                                            \
+          for (int iz = 0; iz < Mz; ++iz) {                                        \
+            T *p = c + (long)nc * ((long)ix * Ny * Nz + iz) * es + comp;           \
+            line_##SFX(t, 1, comp, p, (long)nc * Nz * es, p, (long)nc * Nz * es);  \
+          }                                                                        \
+      if (dim >= 3)                                                                \
+        for (int ix = 0; ix < Nx; ++ix)                                            \
+          for (int iy = 0; iy < Ny; ++iy) {                                        \
+            T *p = c + (long)nc * (((long)ix * Ny + iy) * Nz) * es + comp;         \
+            line_##SFX(t, 2, comp, p, (long)nc * es, p, (long)nc * es);            \
+          }                                                                        \
     }                                                                              \
   }
 
@@ -237,68 +245,94 @@ DEFINE_SOLVERS(double, d)
  * the z sweep then overwrites, so the tables are identical. */
 
 void oracle_set_s(const otable *t, float *coefs, int n, const float *data) {
-  set3d_f(t, coefs, n, data);
+  setnd_f(t, coefs, n, data);
 }
 void oracle_set_d(const otable *t, double *coefs, int n, const double *data) {
-  set3d_d(t, coefs, n, data);
+  setnd_d(t, coefs, n, data);
 }
 void oracle_set_c(const otable *t, float *coefs, int n, const float *data) {
-  set3d_f(t, coefs, n, data);
+  setnd_f(t, coefs, n, data);
 }
 void oracle_set_z(const otable *t, double *coefs, int n, const double *data) {
-  set3d_d(t, coefs, n, data);
+  setnd_d(t, coefs, n, data);
 }
 
 /* ---- evaluation -------------------------------------------------------- */
-/* Returns 0, or -1 (outputs NaN) when the stencil would leave the table,
+/* One loop for every dimension: an axis past dim has the single weight 1 and
+ * zero derivatives, so its products reduce exactly (x*1 == x) to the 1-D and
+ * 2-D reference loops (std_d_impl.h:37-180 and :183-476), and to the 3-D ones
+ * (:482-864) for dim 3.  vghgh follows :868-1049: third-derivative weights
+ * are the constant column of d3A (bspline_data.c:195-200).
+ * Returns 0, or -1 (outputs NaN) when the stencil would leave the table,
  * where the reference reads outside its allocation. */
 #define DEFINE_EVAL(T, SFX, MODF, A, DA, D2A)                                      \
   static int eval_##SFX(const otable *t, const T *coefs, int mode, double x, double y, \
-                        double z, T *vals, T *grads, T *lh) {                      \
-    const int nc = t->ncomp, N = t->N, NR = N * nc;                                \
-    x -= t->start[0]; y -= t->start[1]; z -= t->start[2];                          \
-    T ux = x * t->delta_inv[0], uy = y * t->delta_inv[1], uz = z * t->delta_inv[2]; \
-    T px, py, pz;                                                                  \
-    T tx = MODF(ux, &px), ty = MODF(uy, &py), tz = MODF(uz, &pz);                  \
-    const int ix = (int)px, iy = (int)py, iz = (int)pz;                            \
-    const int ok = ix >= 0 && iy >= 0 && iz >= 0 && ix + 3 < t->Nn[0] &&           \
-                   iy + 3 < t->Nn[1] && iz + 3 < t->Nn[2];                         \
-    const int nh = mode == OR_VGH ? 9 : mode == OR_VGL ? 1 : 0;                     \
+                        double z, T *vals, T *grads, T *lh, T *gh) {               \
+    const int nc = t->ncomp, N = t->N, NR = N * nc, dim = t->dim;                  \
+    const double xx[3] = {x, y, z};                                                \
+    T tt[3] = {0, 0, 0};                                                           \
+    int ii[3] = {0, 0, 0}, ok = 1;                                                 \
+    for (int d = 0; d < dim; ++d) {                                                \
+      T u = (xx[d] - t->start[d]) * t->delta_inv[d], ip;                           \
+      tt[d] = MODF(u, &ip);                                                        \
+      ii[d] = (int)ip;                                                             \
+      ok = ok && ii[d] >= 0 && ii[d] + 3 < t->Nn[d];                               \
+    }                                                                              \
+    const int ng = dim, nh = mode == OR_VGHGH ? 9 : mode == OR_VGH ? (dim == 1 ? 1 : dim * dim) \
+                                                 : mode == OR_VGL ? 1 : 0;         \
+    const int ngh = mode == OR_VGHGH ? 27 : 0;                                     \
     if (!ok) {                                                                     \
       for (int q = 0; q < NR; ++q) vals[q] = (T)NAN;                               \
-      if (mode >= OR_VG) for (int q = 0; q < 3 * NR; ++q) grads[q] = (T)NAN;       \
+      if (mode >= OR_VG) for (int q = 0; q < ng * NR; ++q) grads[q] = (T)NAN;      \
       for (int q = 0; q < nh * NR; ++q) lh[q] = (T)NAN;                            \
+      for (int q = 0; q < ngh * NR; ++q) gh[q] = (T)NAN;                           \
       return -1;                                                                   \
     }                                                                              \
-    T tp[3][4], a[3][4], da[3][4], d2a[3][4];                                      \
-    const T tt[3] = {tx, ty, tz};                                                  \
+    /* weights per axis; an inactive axis has a = {1}, derivatives 0 */          \
+    T a[3][4], da[3][4], d2a[3][4], d3a[3][4];                                     \
+    int nk[3];                                                                     \
     for (int d = 0; d < 3; ++d) {                                                  \
-      tp[d][0] = tt[d] * tt[d] * tt[d]; tp[d][1] = tt[d] * tt[d];                  \
-      tp[d][2] = tt[d]; tp[d][3] = 1;                                              \
+      nk[d] = d < dim ? 4 : 1;                                                     \
+      if (d >= dim) {                                                              \
+        a[d][0] = 1; da[d][0] = 0; d2a[d][0] = 0; d3a[d][0] = 0;                   \
+        continue;                                                                  \
+      }                                                                            \
+      const T tp[4] = {tt[d] * tt[d] * tt[d], tt[d] * tt[d], tt[d], 1};            \
       for (int r = 0; r < 4; ++r) {                                                \
-        a[d][r] = (A[4 * r + 0] * tp[d][0] + A[4 * r + 1] * tp[d][1] +             \
-                   A[4 * r + 2] * tp[d][2] + A[4 * r + 3] * tp[d][3]);             \
-        da[d][r] = (DA[4 * r + 0] * tp[d][0] + DA[4 * r + 1] * tp[d][1] +          \
-                    DA[4 * r + 2] * tp[d][2] + DA[4 * r + 3] * tp[d][3]);          \
-        d2a[d][r] = (D2A[4 * r + 0] * tp[d][0] + D2A[4 * r + 1] * tp[d][1] +       \
-                     D2A[4 * r + 2] * tp[d][2] + D2A[4 * r + 3] * tp[d][3]);       \
+        a[d][r] = (A[4 * r + 0] * tp[0] + A[4 * r + 1] * tp[1] +                   \
+                   A[4 * r + 2] * tp[2] + A[4 * r + 3] * tp[3]);                   \
+        da[d][r] = (DA[4 * r + 0] * tp[0] + DA[4 * r + 1] * tp[1] +                \
+                    DA[4 * r + 2] * tp[2] + DA[4 * r + 3] * tp[3]);                \
+        d2a[d][r] = (D2A[4 * r + 0] * tp[0] + D2A[4 * r + 1] * tp[1] +             \
+                     D2A[4 * r + 2] * tp[2] + D2A[4 * r + 3] * tp[3]);             \
+        d3a[d][r] = (T)D3_COL[r] * tp[3];                                          \
       }                                                                            \
     }                                                                              \
+    /* hessian slot of each unique pair (xx xy xz yy yz zz) in the dim x dim  \
+     * row-major block; 1-D vgh is vgl (std_d_impl.h:168-180) */                 \
+    const int hs3[6] = {0, 1, 2, 4, 5, 8}, hs2[6] = {0, 1, -1, 3, -1, -1};          \
+    const int *hs = dim == 3 ? hs3 : hs2;                                          \
+    /* gradhess slots of the unique triples (xxx xxy xxz xyy xyz xzz yyy yyz  \
+     * yzz zzz), std_d_impl.h:1006-1015 */                                        \
+    const int gs[10] = {0, 1, 2, 4, 5, 8, 13, 14, 17, 26};                         \
     T *l3 = NULL;                                                                  \
-    if (mode == OR_VGL) l3 = calloc((size_t)3 * NR, sizeof(T));                    \
+    const int lap = mode == OR_VGL || (mode == OR_VGH && dim == 1);                \
+    if (lap) l3 = calloc((size_t)3 * NR, sizeof(T));                               \
     for (int q = 0; q < NR; ++q) vals[q] = 0;                                      \
-    if (mode >= OR_VG) for (int q = 0; q < 3 * NR; ++q) grads[q] = 0;              \
-    if (mode == OR_VGH) for (int q = 0; q < 9 * NR; ++q) lh[q] = 0;                \
-    for (int i = 0; i < 4; ++i)                                                    \
-      for (int j = 0; j < 4; ++j)                                                  \
-        for (int k = 0; k < 4; ++k) {                                              \
+    if (mode >= OR_VG) for (int q = 0; q < ng * NR; ++q) grads[q] = 0;             \
+    if (!lap) for (int q = 0; q < nh * NR; ++q) lh[q] = 0;                         \
+    for (int q = 0; q < ngh * NR; ++q) gh[q] = 0;                                  \
+    const int ix = ii[0], iy = ii[1], iz = ii[2];                                  \
+    for (int i = 0; i < nk[0]; ++i)                                                \
+      for (int j = 0; j < nk[1]; ++j)                                              \
+        for (int k = 0; k < nk[2]; ++k) {                                          \
           const T abc = a[0][i] * a[1][j] * a[2][k];                               \
           const T *cf = coefs + ((ix + i) * t->xs + (iy + j) * t->ys + (iz + k) * t->zs) * nc; \
           if (mode == OR_V) {                                                      \
             for (int q = 0; q < NR; ++q) vals[q] += abc * cf[q];                   \
             continue;                                                              \
           }                                                                        \
-          T g[3], h[6];                                                            \
+          T g[3], h[6], g3[10];                                                    \
           g[0] = da[0][i] * a[1][j] * a[2][k];                                     \
           g[1] = a[0][i] * da[1][j] * a[2][k];                                     \
           g[2] = a[0][i] * a[1][j] * da[2][k];                                     \
@@ -308,51 +342,84 @@ void oracle_set_z(const otable *t, double *coefs, int n, const double *data) {
           h[3] = a[0][i] * d2a[1][j] * a[2][k];                                    \
           h[4] = a[0][i] * da[1][j] * da[2][k];                                    \
           h[5] = a[0][i] * a[1][j] * d2a[2][k];                                    \
+          if (mode == OR_VGHGH) {                                                  \
+            g3[0] = d3a[0][i] * a[1][j] * a[2][k];                                 \
+            g3[1] = d2a[0][i] * da[1][j] * a[2][k];                                \
+            g3[2] = d2a[0][i] * a[1][j] * da[2][k];                                \
+            g3[3] = da[0][i] * d2a[1][j] * a[2][k];                                \
+            g3[4] = da[0][i] * da[1][j] * da[2][k];                                \
+            g3[5] = da[0][i] * a[1][j] * d2a[2][k];                                \
+            g3[6] = a[0][i] * d3a[1][j] * a[2][k];                                 \
+            g3[7] = a[0][i] * d2a[1][j] * da[2][k];                                \
+            g3[8] = a[0][i] * da[1][j] * d2a[2][k];                                \
+            g3[9] = a[0][i] * a[1][j] * d3a[2][k];                                 \
+          }                                                                        \
           for (int q = 0; q < NR; ++q) {                                           \
             const int n = q / nc, r = q % nc;                                      \
             const T cv = cf[q];                                                    \
             vals[q] += abc * cv;                                                   \
-            grads[(3 * n + 0) * nc + r] += g[0] * cv;                              \
-            grads[(3 * n + 1) * nc + r] += g[1] * cv;                              \
-            grads[(3 * n + 2) * nc + r] += g[2] * cv;                              \
-            if (mode == OR_VGL) {                                                  \
-              l3[(3 * n + 0) * nc + r] += h[0] * cv;                               \
-              l3[(3 * n + 1) * nc + r] += h[3] * cv;                               \
-              l3[(3 * n + 2) * nc + r] += h[5] * cv;                               \
-            } else if (mode == OR_VGH) {                                           \
-              lh[(9 * n + 0) * nc + r] += h[0] * cv;                               \
-              lh[(9 * n + 1) * nc + r] += h[1] * cv;                               \
-              lh[(9 * n + 2) * nc + r] += h[2] * cv;                               \
-              lh[(9 * n + 4) * nc + r] += h[3] * cv;                               \
-              lh[(9 * n + 5) * nc + r] += h[4] * cv;                               \
-              lh[(9 * n + 8) * nc + r] += h[5] * cv;                               \
+            for (int d = 0; d < ng; ++d) grads[(ng * n + d) * nc + r] += g[d] * cv; \
+            if (lap) {                                                             \
+              for (int d = 0; d < dim; ++d)                                        \
+                l3[(3 * n + d) * nc + r] += h[d == 0 ? 0 : d == 1 ? 3 : 5] * cv;    \
+            } else if (mode >= OR_VGH) {                                           \
+              for (int u = 0; u < 6; ++u)                                          \
+                if (hs[u] >= 0) lh[(nh * n + hs[u]) * nc + r] += h[u] * cv;        \
             }                                                                      \
+            if (mode == OR_VGHGH)                                                  \
+              for (int u = 0; u < 10; ++u) gh[(27 * n + gs[u]) * nc + r] += g3[u] * cv; \
           }                                                                        \
         }                                                                          \
     if (mode >= OR_VG) {                                                           \
-      const T dx = t->delta_inv[0], dy = t->delta_inv[1], dz = t->delta_inv[2];    \
+      const T di[3] = {(T)t->delta_inv[0], (T)t->delta_inv[1], (T)t->delta_inv[2]}; \
       for (int q = 0; q < NR; ++q) {                                               \
         const int n = q / nc, r = q % nc;                                          \
-        grads[(3 * n + 0) * nc + r] *= dx;                                         \
-        grads[(3 * n + 1) * nc + r] *= dy;                                         \
-        grads[(3 * n + 2) * nc + r] *= dz;                                         \
-        if (mode == OR_VGL) {                                                      \
-          l3[(3 * n + 0) * nc + r] *= dx * dx;                                     \
-          l3[(3 * n + 1) * nc + r] *= dy * dy;                                     \
-          l3[(3 * n + 2) * nc + r] *= dz * dz;                                     \
-          lh[q] = l3[(3 * n + 0) * nc + r] + l3[(3 * n + 1) * nc + r] +            \
-                  l3[(3 * n + 2) * nc + r];                                        \
-        } else if (mode == OR_VGH) {                                               \
+        for (int d = 0; d < ng; ++d) grads[(ng * n + d) * nc + r] *= di[d];        \
+        if (lap) {                                                                 \
+          T sum = 0;                                                               \
+          for (int d = 0; d < dim; ++d) {                                          \
+            l3[(3 * n + d) * nc + r] *= di[d] * di[d];                             \
+            sum = d == 0 ? l3[(3 * n) * nc + r] : sum + l3[(3 * n + d) * nc + r];  \
+          }                                                                        \
+          lh[q] = sum;                                                             \
+        } else if (mode >= OR_VGH && dim == 3) {                                   \
           T *hh = lh + 9 * n * nc + r;                                             \
-          hh[0 * nc] *= dx * dx;                                                   \
-          hh[4 * nc] *= dy * dy;                                                   \
-          hh[8 * nc] *= dz * dz;                                                   \
-          hh[1 * nc] *= dx * dy;                                                   \
-          hh[2 * nc] *= dx * dz;                                                   \
-          hh[5 * nc] *= dy * dz;                                                   \
+          hh[0 * nc] *= di[0] * di[0];                                             \
+          hh[4 * nc] *= di[1] * di[1];                                             \
+          hh[8 * nc] *= di[2] * di[2];                                             \
+          hh[1 * nc] *= di[0] * di[1];                                             \
+          hh[2 * nc] *= di[0] * di[2];                                             \
+          hh[5 * nc] *= di[1] * di[2];                                             \
           hh[3 * nc] = hh[1 * nc];                                                 \
           hh[6 * nc] = hh[2 * nc];                                                 \
           hh[7 * nc] = hh[5 * nc];                                                 \
+        } else if (mode == OR_VGH && dim == 2) {                                   \
+          T *hh = lh + 4 * n * nc + r;                                             \
+          hh[0 * nc] *= di[0] * di[0];                                             \
+          hh[1 * nc] *= di[0] * di[1];                                             \
+          hh[3 * nc] *= di[1] * di[1];                                             \
+          hh[2 * nc] = hh[1 * nc];                                                 \
+        }                                                                          \
+        if (mode == OR_VGHGH) {                                                    \
+          T *G = gh + 27 * n * nc + r;                                             \
+          G[0 * nc] *= di[0] * di[0] * di[0];                                      \
+          G[1 * nc] *= di[0] * di[0] * di[1];                                      \
+          G[2 * nc] *= di[0] * di[0] * di[2];                                      \
+          G[4 * nc] *= di[0] * di[1] * di[1];                                      \
+          G[5 * nc] *= di[0] * di[1] * di[2];                                      \
+          G[8 * nc] *= di[0] * di[2] * di[2];                                      \
+          G[13 * nc] *= di[1] * di[1] * di[1];                                     \
+          G[14 * nc] *= di[1] * di[1] * di[2];                                     \
+          G[17 * nc] *= di[1] * di[2] * di[2];                                     \
+          G[26 * nc] *= di[2] * di[2] * di[2];                                     \
+          /* every slot (a,b,c) holds the entry of the sorted triple */           \
+          for (int u = 0; u < 27; ++u) {                                           \
+            int e[3] = {u / 9, (u / 3) % 3, u % 3}, tmp;                           \
+            if (e[0] > e[1]) { tmp = e[0]; e[0] = e[1]; e[1] = tmp; }              \
+            if (e[1] > e[2]) { tmp = e[1]; e[1] = e[2]; e[2] = tmp; }              \
+            if (e[0] > e[1]) { tmp = e[0]; e[0] = e[1]; e[1] = tmp; }              \
+            G[u * nc] = G[(9 * e[0] + 3 * e[1] + e[2]) * nc];                      \
+          }                                                                        \
         }                                                                          \
       }                                                                            \
     }                                                                              \
@@ -360,42 +427,51 @@ void oracle_set_z(const otable *t, double *coefs, int n, const double *data) {
     return 0;                                                                      \
   }
 
+/* constant column of the third-derivative basis (src/bspline_data.c:195-200) */
+static const double D3_COL[4] = {-1.0, 3.0, -3.0, 1.0};
+
 DEFINE_EVAL(float, f, modff, A_f, dA_f, d2A_f)
 DEFINE_EVAL(double, d, modf, A_d, dA_d, d2A_d)
 
 int oracle_eval_s(const otable *t, const float *coefs, int mode, double x, double y, double z,
-                  float *v, float *g, float *lh) {
-  return eval_f(t, coefs, mode, x, y, z, v, g, lh);
+                  float *v, float *g, float *lh, float *gh) {
+  return eval_f(t, coefs, mode, x, y, z, v, g, lh, gh);
 }
 int oracle_eval_d(const otable *t, const double *coefs, int mode, double x, double y, double z,
-                  double *v, double *g, double *lh) {
-  return eval_d(t, coefs, mode, x, y, z, v, g, lh);
+                  double *v, double *g, double *lh, double *gh) {
+  return eval_d(t, coefs, mode, x, y, z, v, g, lh, gh);
 }
 
-/* Many points, OpenMP over points.  Outputs of point p start at
- * vals + p*N*nc, grads + p*3*N*nc, lh + p*{1,9}*N*nc. */
+/* Many points, OpenMP over points.  Point p reads pos[dim*p ...] and writes
+ * vals + p*NR, grads + p*dim*NR, lh + p*nh*NR, gh + p*27*NR (NR = N*ncomp,
+ * nh as in eval_). */
 int oracle_eval_batch(const otable *t, int is_double, const void *coefs, int mode,
                       const double *pos, int npts, void *vals, void *grads, void *lh,
-                      int nthreads) {
+                      void *gh, int nthreads) {
   const long NR = (long)t->N * t->ncomp;
-  const long nh = mode == OR_VGH ? 9 : mode == OR_VGL ? 1 : 0;
+  const int dim = t->dim;
+  const long nh = mode == OR_VGHGH ? 9 : mode == OR_VGH ? (dim == 1 ? 1 : dim * dim)
+                : mode == OR_VGL ? 1 : 0;
   int bad = 0;
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : bad)
 #endif
   for (int p = 0; p < npts; ++p) {
-    const double *x = pos + 3l * p;
+    double x[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) x[d] = pos[(long)dim * p + d];
     if (is_double) {
       double *v = (double *)vals + p * NR;
-      double *g = grads ? (double *)grads + 3 * p * NR : NULL;
+      double *g = grads ? (double *)grads + dim * p * NR : NULL;
       double *h = lh ? (double *)lh + nh * p * NR : NULL;
-      bad += eval_d(t, (const double *)coefs, mode, x[0], x[1], x[2], v, g, h) != 0;
+      double *G = gh ? (double *)gh + 27 * p * NR : NULL;
+      bad += eval_d(t, (const double *)coefs, mode, x[0], x[1], x[2], v, g, h, G) != 0;
     } else {
       float *v = (float *)vals + p * NR;
-      float *g = grads ? (float *)grads + 3 * p * NR : NULL;
+      float *g = grads ? (float *)grads + dim * p * NR : NULL;
       float *h = lh ? (float *)lh + nh * p * NR : NULL;
-      bad += eval_f(t, (const float *)coefs, mode, x[0], x[1], x[2], v, g, h) != 0;
+      float *G = gh ? (float *)gh + 27 * p * NR : NULL;
+      bad += eval_f(t, (const float *)coefs, mode, x[0], x[1], x[2], v, g, h, G) != 0;
     }
   }
   (void)nthreads;
@@ -410,6 +486,10 @@ void oracle_basis_d(double tt, double a[4], double da[4], double d2a[4]) {
     da[r] = dA_d[4 * r] * tp[0] + dA_d[4 * r + 1] * tp[1] + dA_d[4 * r + 2] * tp[2] + dA_d[4 * r + 3] * tp[3];
     d2a[r] = d2A_d[4 * r] * tp[0] + d2A_d[4 * r + 1] * tp[1] + d2A_d[4 * r + 2] * tp[2] + d2A_d[4 * r + 3] * tp[3];
   }
+}
+
+void oracle_basis3_d(double d3a[4]) {
+  for (int r = 0; r < 4; ++r) d3a[r] = D3_COL[r];
 }
 
 int oracle_max_threads(void) {

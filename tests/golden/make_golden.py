@@ -39,7 +39,7 @@ def case(name, kind, M, start, end, N, bcs, seed, P=24):
         co = co.astype(np.complex64 if cplx else np.float32)
     s, e = np.array(start), np.array(end)
     pts = s + (e - s) * rng.random((P, 3)) * 0.999999
-    v, g, H = brute.eval_dense(co.astype(np.complex128 if cplx else np.float64), start, dinv, pts)
+    v, g, H, _ = brute.eval_dense(co.astype(np.complex128 if cplx else np.float64), start, dinv, pts)
     if cplx:
         co_store = np.ascontiguousarray(co).view(np.float32 if kind == "c" else np.float64)
     else:

@@ -1,0 +1,138 @@
+"""Locality traversal (einspline_set_point_order): batches sorted by cell give
+bitwise the same results as the given order, for every entry point that
+takes a batch, on several streams, across workspace growth, on grids large
+enough to need coarsened sort keys, and with points outside the box."""
+import numpy as np
+import pytest
+
+import einspline_amd as ea
+import oracle
+from test_gpu_parity import TOL, gpu_eval, make_pair, normwise, points
+
+pytestmark = pytest.mark.gpu
+
+
+def both_orders(tab, mode, pts, packed=False):
+    tab.set_point_order(ea.ORDER_GIVEN)
+    given = gpu_eval(tab, mode, pts, packed)
+    tab.set_point_order(ea.ORDER_LOCALITY)
+    local = gpu_eval(tab, mode, pts, packed)
+    tab.set_point_order(ea.ORDER_GIVEN)
+    return given, local
+
+
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+@pytest.mark.parametrize("mode", ["v", "vg", "vgl", "vgh"])
+def test_locality_order_is_bitwise_identical(kind, mode, rng):
+    h, o = make_pair(kind, (12, 10, 9), 64, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 3000)
+    given, local = both_orders(tab, mode, pts)
+    for a, b in zip(local, given):
+        np.testing.assert_array_equal(a, b)
+    ref = o.eval_batch(mode, pts[:300])
+    for a, r in zip(local, ref):
+        assert normwise(a[:300], r) < TOL[kind]
+
+
+def test_locality_order_with_tiles_policies_and_packed(rng):
+    h, _ = make_pair("s", (16, 16, 16), 256, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 4100)
+    base = gpu_eval(tab, "vgh", pts)
+    base_p = gpu_eval(tab, "vgh", pts, packed=True)
+    tab.set_point_order(ea.ORDER_LOCALITY)
+    for pol in (0, 1, 2):
+        tab.set_store_policy(pol)
+        for lb, blk in ((8, 256), (16, 64), (32, 128)):
+            tab.set_tile(lb, blk)
+            for a, b in zip(gpu_eval(tab, "vgh", pts), base):
+                np.testing.assert_array_equal(a, b)
+    for a, b in zip(gpu_eval(tab, "vgh", pts, packed=True), base_p):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_points_outside_the_box_and_nan(rng):
+    """Keys clamp like the kernel's cell index; a NaN position sorts into
+    cell 0 and still gets exactly the given-order result."""
+    h, _ = make_pair("d", (8, 8, 8), 16, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 2000)
+    pts[::7] = pts[::7] * 5 - 2            # far outside on both sides
+    pts[5] = [np.nan, 0.5, 0.5]
+    given, local = both_orders(tab, "vgh", pts)
+    for a, b in zip(local, given):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_huge_axis_uses_coarsened_keys(rng):
+    """An axis of more than 1024 cells needs the key shift; results unchanged."""
+    h, o = make_pair("s", (1500, 5, 4), 8, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 5000)
+    given, local = both_orders(tab, "vgl", pts)
+    for a, b in zip(local, given):
+        np.testing.assert_array_equal(a, b)
+    ref = o.eval_batch("vgl", pts[:100])
+    for a, r in zip(local, ref):
+        assert normwise(a[:100], r) < TOL["s"]
+
+
+def test_workspace_growth_and_two_streams(rng):
+    h, _ = make_pair("s", (14, 14, 14), 128, rng)
+    tab = ea.DeviceTable(h)
+    ref_tab = ea.DeviceTable(h)
+    streams = [ea.Stream(), ea.Stream()]
+    tab.set_point_order(ea.ORDER_LOCALITY)
+    jobs = []
+    for i, P in enumerate((1500, 6000, 1100, 20000, 3000, 900)):
+        pts = points(rng, P)
+        pos = ea.DeviceBuffer.from_array(pts)
+        outs = tab.alloc_outputs("vgh", P)
+        tab.eval_batch("vgh", pos, P, outs, stream=streams[i % 2])
+        jobs.append((pts, pos, outs))
+    for s in streams:
+        s.synchronize()
+    for pts, pos, outs in jobs:
+        got = tab.download("vgh", len(pts), outs)
+        want = gpu_eval(ref_tab, "vgh", pts)
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a.reshape(b.shape), b)
+
+
+@pytest.mark.parametrize("kind", ["s", "z"])
+def test_pointer_array_api_in_locality_order(kind, rng):
+    M, N, P = (9, 9, 9), 32, 2500
+    h, _ = make_pair(kind, M, N, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, P)
+    fpos = kind in "sc"
+    pos = ea.DeviceBuffer.from_array(pts.astype(np.float32) if fpos else pts)
+    nr = tab.nreal
+    es = np.dtype(tab.result_dtype).itemsize
+    f = getattr(ea.lib, f"eval_multi_multi_UBspline_3d_{kind}_vgh_hip")
+    res = []
+    for order in (ea.ORDER_GIVEN, ea.ORDER_LOCALITY):
+        tab.set_point_order(order)
+        vb, gb, hb = (ea.DeviceBuffer(P * k * nr * es) for k in (1, 3, 6))
+        ptrs = lambda b, k: ea.DeviceBuffer.from_array(
+            np.array([b.ptr + p * k * nr * es for p in range(P)], np.uint64))
+        vp, gp, hp = ptrs(vb, 1), ptrs(gb, 3), ptrs(hb, 6)
+        ea.check(f(tab.ptr, pos.ptr, vp.ptr, gp.ptr, hp.ptr, P), "vgh_hip")
+        res.append([b.download(tab.result_dtype, (P, k * nr)) for b, k in ((vb, 1), (gb, 3), (hb, 6))])
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_autotune_reports_an_order_and_keeps_results(rng):
+    h, _ = make_pair("s", (20, 20, 20), 128, rng)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 3000)
+    before = gpu_eval(tab, "vgh", pts)
+    tab.autotune("vgh", 4096)
+    lb, blk, pol, order = tab.tuning()
+    assert order in (ea.ORDER_GIVEN, ea.ORDER_LOCALITY)
+    for a, b in zip(gpu_eval(tab, "vgh", pts), before):
+        np.testing.assert_array_equal(a, b)
+    with pytest.raises(ea.EinsplineError):
+        tab.set_point_order(7)

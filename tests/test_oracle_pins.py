@@ -131,7 +131,7 @@ def test_eval_d_matches_brute_force(bcs, rng):
     t = oracle.Table("d", M, start, end, N, bc=bcs)
     t.coefs[...] = rng.uniform(-1, 1, t.coefs.shape)    # arbitrary coefficients
     pts = random_points(rng, start, end, 40)
-    v, g, H = brute.eval_dense(t.coefs, start, t.delta_inv, pts)
+    v, g, H, _ = brute.eval_dense(t.coefs, start, t.delta_inv, pts)
     ov, og, oh = t.eval_batch("vgh", pts)
     assert rel(ov, v) < 1e-13 and rel(og, g) < 1e-13 and rel(oh, H) < 1e-13
     ov2, og2, ol = t.eval_batch("vgl", pts)
