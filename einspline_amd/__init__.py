@@ -30,9 +30,9 @@ lib = C.CDLL(LIB_PATH)
 
 PERIODIC, DERIV1, DERIV2, FLAT, NATURAL, ANTIPERIODIC = range(6)
 SINGLE_REAL, DOUBLE_REAL, SINGLE_COMPLEX, DOUBLE_COMPLEX = range(4)
-MULTI_U3D = 8
+MULTI_U1D, MULTI_U2D, MULTI_U3D = 6, 7, 8
 STORE_F32, STORE_F64, STORE_BF16, STORE_F16 = range(4)
-MODES = ("v", "vg", "vgl", "vgh")
+MODES = ("v", "vg", "vgl", "vgh")          # autotuner modes 0..3; "vghgh" is batch-only
 ORDER_GIVEN, ORDER_LOCALITY = 0, 1
 
 E_OK, E_INVAL, E_NOMEM, E_HIP, E_RANGE, E_BUDGET, E_BUSY = 0, -1, -2, -3, -4, -5, -6
@@ -87,6 +87,26 @@ class multi_UBspline_3d_z(C.Structure):
     _fields_ = _multi(BCtype_z, True)
 
 
+def _multi12(bc, dim, lapl2):
+    axes = "xy"[:dim]
+    f = [("spcode", C.c_int), ("tcode", C.c_int), ("coefs", C.c_void_p)]
+    f += [(a + "_stride", C.c_ssize_t) for a in axes]
+    f += [(a + "_grid", Ugrid) for a in axes]
+    f += [(a + "BC", bc) for a in axes]
+    f += [("num_splines", C.c_int)]
+    if lapl2:
+        f.append(("lapl2", C.c_void_p))
+    return f
+
+
+# 1-D and 2-D descriptors, keyed (dim, kind)
+STRUCTS12 = {}
+for _d in (1, 2):
+    for _k, _bc in (("s", BCtype_s), ("d", BCtype_d), ("c", BCtype_c), ("z", BCtype_z)):
+        STRUCTS12[(_d, _k)] = type(f"multi_UBspline_{_d}d_{_k}", (C.Structure,),
+                                   {"_fields_": _multi12(_bc, _d, _d == 2 and _k in "cz")})
+
+
 class multi_UBspline_3d_hip(C.Structure):
     _fields_ = [("coefs", C.c_void_p),
                 ("x_stride", C.c_ssize_t), ("y_stride", C.c_ssize_t), ("z_stride", C.c_ssize_t),
@@ -94,7 +114,7 @@ class multi_UBspline_3d_hip(C.Structure):
                 ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int),
                 ("num_splines", C.c_int), ("row_reals", C.c_int),
                 ("tcode", C.c_int), ("storage", C.c_int),
-                ("bytes", C.c_size_t), ("device", C.c_int)]
+                ("bytes", C.c_size_t), ("device", C.c_int), ("dim", C.c_int)]
 
 
 KINDS = {
@@ -126,7 +146,28 @@ for k, (S, BC, RT, _, _) in KINDS.items():
     _proto(f"eval_multi_UBspline_3d_{k}_vg", None, C.POINTER(S), _d, _d, _d, _vp, _vp)
     _proto(f"eval_multi_UBspline_3d_{k}_vgl", None, C.POINTER(S), _d, _d, _d, _vp, _vp, _vp)
     _proto(f"eval_multi_UBspline_3d_{k}_vgh", None, C.POINTER(S), _d, _d, _d, _vp, _vp, _vp)
+    _proto(f"eval_multi_UBspline_3d_{k}_vghgh", None, C.POINTER(S), _d, _d, _d, _vp, _vp, _vp, _vp)
     _proto(f"create_multi_UBspline_3d_{k}_hip", C.POINTER(multi_UBspline_3d_hip), C.POINTER(S))
+    for d in (1, 2):
+        S12 = STRUCTS12[(d, k)]
+        _proto(f"create_multi_UBspline_{d}d_{k}", C.POINTER(S12), *([Ugrid] * d), *([BC] * d), _i)
+        _proto(f"set_multi_UBspline_{d}d_{k}", None, C.POINTER(S12), _i, _vp)
+        _proto(f"create_multi_UBspline_{d}d_{k}_hip", C.POINTER(multi_UBspline_3d_hip), C.POINTER(S12))
+        for m, nout in (("", 1), ("_vg", 2), ("_vgl", 3), ("_vgh", 3)):
+            _proto(f"eval_multi_UBspline_{d}d_{k}{m}", None, C.POINTER(S12), *([_d] * d), *([_vp] * nout))
+for d in (1, 2):
+    _proto(f"create_multi_UBspline_{d}d_s_hip_conv", C.POINTER(multi_UBspline_3d_hip),
+           C.POINTER(STRUCTS12[(d, "d")]))
+    _proto(f"create_multi_UBspline_{d}d_c_hip_conv", C.POINTER(multi_UBspline_3d_hip),
+           C.POINTER(STRUCTS12[(d, "z")]))
+_proto("set_multi_UBspline_1d_d_BC", None, C.POINTER(STRUCTS12[(1, "d")]), _i, _vp, BCtype_d)
+_proto("set_multi_UBspline_1d_z_BC", None, C.POINTER(STRUCTS12[(1, "z")]), _i, _vp, BCtype_z)
+for k in "sdcz":
+    _proto(f"eval_multi_multi_UBspline_1d_{k}_hip", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _i)
+_proto("eval_multi_multi_UBspline_1d_s_vgl_hip", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _vp, _vp, _i)
+for k in "dcz":
+    _proto(f"eval_multi_multi_UBspline_1d_{k}_vgl_hip", _i, C.POINTER(multi_UBspline_3d_hip),
+           _vp, _vp, _vp, _i, _i)
     _proto(f"eval_multi_multi_UBspline_3d_{k}_hip", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _i)
     _proto(f"eval_multi_multi_UBspline_3d_{k}_vgh_hip", _i, C.POINTER(multi_UBspline_3d_hip),
            _vp, _vp, _vp, _vp, _i)
@@ -156,6 +197,7 @@ _proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, 
 _proto("einspline_eval_vg_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp)
 _proto("einspline_eval_vgl_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp)
 _proto("einspline_eval_vgh_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _i, _vp)
+_proto("einspline_eval_vghgh_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp, _vp)
 _proto("einspline_hip_device_count", _i)
 _proto("einspline_hip_last_error", C.c_char_p)
 _proto("einspline_hip_malloc", _vp, _sz)
@@ -173,6 +215,7 @@ _proto("einspline_hip_event_record", _i, _vp, _vp)
 _proto("einspline_hip_event_synchronize", _i, _vp)
 _proto("einspline_hip_event_elapsed_ms", C.c_float, _vp, _vp)
 _proto("einspline_hip_stream_copy", _i, _vp, _vp, _sz, _vp)
+_proto("einspline_hip_stream_fill", _i, _vp, C.c_uint, _sz, _i, _vp)
 
 
 class EinsplineError(RuntimeError):
@@ -297,22 +340,24 @@ def _bc(kind, lcode, rcode, lval=0.0, rval=0.0):
 
 
 class HostSpline:
-    """A multi_UBspline_3d_{s,d,c,z} created by libeinspline.
+    """A multi_UBspline_{1,2,3}d_{s,d,c,z} created by libeinspline.
 
-    M, start, end: per-axis grid (3-sequences); bc: per-axis
-    (lcode, rcode[, lval, rval]) tuples, default periodic.
+    M, start, end: per-axis grid (sequences of 1, 2 or 3; their length is
+    the dimension); bc: per-axis (lcode, rcode[, lval, rval]) tuples,
+    default periodic.
     """
 
     def __init__(self, kind, M, start, end, num_splines, bc=None):
         S, BC, RT, dt, nc = KINDS[kind]
         self.kind, self.dtype, self.ncomp = kind, dt, nc
+        self.dim = dim = len(M)
         self.N = int(num_splines)
-        bc = bc or [(PERIODIC, PERIODIC)] * 3
+        bc = bc or [(PERIODIC, PERIODIC)] * dim
         grids = [Ugrid(float(s), float(e), int(m), 0.0, 0.0) for m, s, e in zip(M, start, end)]
         bcs = [_bc(kind, *b) for b in bc]
-        self.ptr = getattr(lib, f"create_multi_UBspline_3d_{kind}")(*grids, *bcs, self.N)
+        self.ptr = getattr(lib, f"create_multi_UBspline_{dim}d_{kind}")(*grids, *bcs, self.N)
         if not self.ptr:
-            raise EinsplineError(E_INVAL, f"create_multi_UBspline_3d_{kind}")
+            raise EinsplineError(E_INVAL, f"create_multi_UBspline_{dim}d_{kind}")
         st = self.ptr.contents
         self.M = tuple(int(m) for m in M)
         self.shape = tuple(int(m) + (3 if b[0] in (PERIODIC, ANTIPERIODIC) else 2) for m, b in zip(M, bc))
@@ -332,11 +377,15 @@ class HostSpline:
 
     def set(self, n, data):
         data = self._data(data, 1)
-        getattr(lib, f"set_multi_UBspline_3d_{self.kind}")(self.ptr, int(n), data.ctypes.data)
+        getattr(lib, f"set_multi_UBspline_{self.dim}d_{self.kind}")(self.ptr, int(n), data.ctypes.data)
 
     def set_block(self, first, data):
-        """data: (count, Mx, My, Mz) real or complex array."""
+        """data: (count, M_0, .., M_{dim-1}) real or complex array."""
         count = data.shape[0]
+        if self.dim != 3:
+            for o in range(count):
+                self.set(first + o, data[o])
+            return
         data = self._data(data, count)
         getattr(lib, f"set_multi_UBspline_3d_{self.kind}_block")(self.ptr, int(first), int(count),
                                                                  data.ctypes.data)
@@ -354,25 +403,32 @@ class HostSpline:
     def invalidate(self):
         lib.einspline_hip_invalidate(self.ptr)
 
-    def eval(self, mode, x, y, z):
-        """Single-point evaluation through eval_multi_UBspline_3d_<kind>[_mode]."""
-        N, nc = self.N, self.ncomp
+    def eval(self, mode, *x):
+        """Single-point evaluation through eval_multi_UBspline_<dim>d_<kind>[_mode];
+        x holds dim coordinates."""
+        N, D = self.N, self.dim
+        assert len(x) == D, f"{D} coordinates"
         cdt = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}[self.kind]
+        nh = {3: 9, 2: 4, 1: 1}[D]
         v = np.zeros(N, cdt)
-        g = np.zeros(3 * N, cdt)
-        h = np.zeros(9 * N if mode == "vgh" else N, cdt)
+        g = np.zeros(D * N, cdt)
+        h = np.zeros(nh * N if mode in ("vgh", "vghgh") else N, cdt)
+        t3 = np.zeros(27 * N, cdt)
         suf = "" if mode == "v" else "_" + mode
-        f = getattr(lib, f"eval_multi_UBspline_3d_{self.kind}{suf}")
+        f = getattr(lib, f"eval_multi_UBspline_{D}d_{self.kind}{suf}")
         args = [v.ctypes.data] + ([g.ctypes.data] if mode != "v" else []) + \
-               ([h.ctypes.data] if mode in ("vgl", "vgh") else [])
-        f(self.ptr, float(x), float(y), float(z), *args)
+               ([h.ctypes.data] if mode in ("vgl", "vgh", "vghgh") else []) + \
+               ([t3.ctypes.data] if mode == "vghgh" else [])
+        f(self.ptr, *[float(c) for c in x], *args)
         out = [v]
         if mode != "v":
-            out.append(g.reshape(N, 3))
-        if mode == "vgl":
+            out.append(g.reshape(N, D))
+        if mode == "vgl" or (mode == "vgh" and D == 1):
             out.append(h)
-        if mode == "vgh":
-            out.append(h.reshape(N, 3, 3))
+        elif mode in ("vgh", "vghgh"):
+            out.append(h.reshape(N, D, D))
+        if mode == "vghgh":
+            out.append(t3.reshape(N, 3, 3, 3))
         return tuple(out)
 
     def destroy(self):
@@ -399,12 +455,12 @@ class DeviceTable:
     """
 
     def __init__(self, host: HostSpline, storage=None):
-        k = host.kind
+        k, D = host.kind, host.dim
         if storage is None:
-            p = getattr(lib, f"create_multi_UBspline_3d_{k}_hip")(host.ptr)
+            p = getattr(lib, f"create_multi_UBspline_{D}d_{k}_hip")(host.ptr)
         elif storage == "conv":
-            p = getattr(lib, {"d": "create_multi_UBspline_3d_s_hip_conv",
-                              "z": "create_multi_UBspline_3d_c_hip_conv"}[k])(host.ptr)
+            p = getattr(lib, {"d": f"create_multi_UBspline_{D}d_s_hip_conv",
+                              "z": f"create_multi_UBspline_{D}d_c_hip_conv"}[k])(host.ptr)
         else:
             assert k == "d", "storage conversion starts from a double table"
             p = lib.create_multi_UBspline_3d_s_hip_store(host.ptr, int(storage))
@@ -419,6 +475,7 @@ class DeviceTable:
         self.result_dtype = np.float64 if t.storage == STORE_F64 else np.float32
         self.nreal = self.N * (2 if self.complex else 1)
         self.bytes = t.bytes
+        self.dim = t.dim
 
     def set_tile(self, lane_bytes=0, block=0):
         check(lib.einspline_set_tile(self.ptr, int(lane_bytes), int(block)), "einspline_set_tile")
@@ -447,10 +504,11 @@ class DeviceTable:
         return lb.value, blk.value, nt.value, ms.value
 
     def out_sizes(self, mode, packed=False):
-        """Reals per point of (vals, grads, lapl/hess)."""
-        n = self.nreal
-        h = {"v": 0, "vg": 0, "vgl": n, "vgh": (6 if packed else 9) * n}[mode]
-        return n, (3 * n if mode != "v" else 0), h
+        """Reals per point of (vals, grads, lapl/hess, third derivatives)."""
+        n, D = self.nreal, self.dim
+        hv = {3: (6 if packed else 9), 2: 4, 1: 1}[D]
+        h = {"v": 0, "vg": 0, "vgl": n, "vgh": hv * n, "vghgh": 9 * n}[mode]
+        return n, (D * n if mode != "v" else 0), h, (27 * n if mode == "vghgh" else 0)
 
     def alloc_outputs(self, mode, npts, packed=False):
         es = np.dtype(self.result_dtype).itemsize
@@ -458,7 +516,7 @@ class DeviceTable:
                      for s in self.out_sizes(mode, packed))
 
     def eval_batch(self, mode, pos: DeviceBuffer, npts, outs, packed=False, stream=None):
-        v, g, h = outs
+        v, g, h, gh = (tuple(outs) + (None,) * 4)[:4]
         sp = stream.ptr if stream else None
         if mode == "v":
             rc = lib.einspline_eval_v_batch(self.ptr, pos.ptr, npts, v.ptr, sp)
@@ -466,6 +524,8 @@ class DeviceTable:
             rc = lib.einspline_eval_vg_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, sp)
         elif mode == "vgl":
             rc = lib.einspline_eval_vgl_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, h.ptr, sp)
+        elif mode == "vghgh":
+            rc = lib.einspline_eval_vghgh_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, h.ptr, gh.ptr, sp)
         else:
             rc = lib.einspline_eval_vgh_batch(self.ptr, pos.ptr, npts, v.ptr, g.ptr, h.ptr,
                                               1 if packed else 0, sp)

@@ -1,4 +1,5 @@
-// cxx_abi.cpp -- C++-linkage copies of the single-point eval functions.
+// cxx_abi.cpp -- C++-linkage copies of the single-point eval functions
+// (1-D, 2-D and 3-D).
 //
 // The reference declares eval_multi_UBspline_3d_* without extern "C"
 // (src/multi_bspline_eval_d.h:31-110) and compiles each implementation a
@@ -76,3 +77,62 @@ void eval_multi_UBspline_3d_z_vgh(multi_UBspline_3d_z *s, double x, double y, do
                                  std::complex<double> *h) {
   c_eval_z_vgh(s, x, y, z, v, g, h);
 }
+
+// vghgh (src/multi_bspline_eval_std_d_impl.h:868 and the s/c/z files)
+#define ESK_VGHGH_FWD(T, TY)                                                                        \
+  void c_eval_##T##_vghgh(multi_UBspline_3d_##T *, double, double, double, TY *, TY *, TY *, TY *)  \
+      __asm__("eval_multi_UBspline_3d_" #T "_vghgh");                                              \
+  void eval_multi_UBspline_3d_##T##_vghgh(multi_UBspline_3d_##T *s, double x, double y, double z,   \
+                                          TY *v, TY *g, TY *h, TY *gh) {                            \
+    c_eval_##T##_vghgh(s, x, y, z, v, g, h, gh);                                                    \
+  }
+ESK_VGHGH_FWD(s, float)
+ESK_VGHGH_FWD(d, double)
+ESK_VGHGH_FWD(c, std::complex<float>)
+ESK_VGHGH_FWD(z, std::complex<double>)
+
+// 1-D and 2-D (src/multi_bspline_eval_std_d_impl.h:37-476 and the s/c/z files,
+// compiled as C++ by the same *_cpp.cc units)
+#define ESK_FWD12(T, TY)                                                                            \
+  void c_eval_1d_##T(multi_UBspline_1d_##T *, double, TY *) __asm__("eval_multi_UBspline_1d_" #T);  \
+  void c_eval_1d_##T##_vg(multi_UBspline_1d_##T *, double, TY *, TY *)                              \
+      __asm__("eval_multi_UBspline_1d_" #T "_vg");                                                  \
+  void c_eval_1d_##T##_vgl(multi_UBspline_1d_##T *, double, TY *, TY *, TY *)                       \
+      __asm__("eval_multi_UBspline_1d_" #T "_vgl");                                                 \
+  void c_eval_1d_##T##_vgh(multi_UBspline_1d_##T *, double, TY *, TY *, TY *)                       \
+      __asm__("eval_multi_UBspline_1d_" #T "_vgh");                                                 \
+  void c_eval_2d_##T(multi_UBspline_2d_##T *, double, double, TY *) __asm__("eval_multi_UBspline_2d_" #T); \
+  void c_eval_2d_##T##_vg(multi_UBspline_2d_##T *, double, double, TY *, TY *)                      \
+      __asm__("eval_multi_UBspline_2d_" #T "_vg");                                                  \
+  void c_eval_2d_##T##_vgl(multi_UBspline_2d_##T *, double, double, TY *, TY *, TY *)               \
+      __asm__("eval_multi_UBspline_2d_" #T "_vgl");                                                 \
+  void c_eval_2d_##T##_vgh(multi_UBspline_2d_##T *, double, double, TY *, TY *, TY *)               \
+      __asm__("eval_multi_UBspline_2d_" #T "_vgh");                                                 \
+  void eval_multi_UBspline_1d_##T(multi_UBspline_1d_##T *s, double x, TY *v) { c_eval_1d_##T(s, x, v); } \
+  void eval_multi_UBspline_1d_##T##_vg(multi_UBspline_1d_##T *s, double x, TY *v, TY *g) {          \
+    c_eval_1d_##T##_vg(s, x, v, g);                                                                 \
+  }                                                                                                 \
+  void eval_multi_UBspline_1d_##T##_vgl(multi_UBspline_1d_##T *s, double x, TY *v, TY *g, TY *l) {  \
+    c_eval_1d_##T##_vgl(s, x, v, g, l);                                                             \
+  }                                                                                                 \
+  void eval_multi_UBspline_1d_##T##_vgh(multi_UBspline_1d_##T *s, double x, TY *v, TY *g, TY *h) {  \
+    c_eval_1d_##T##_vgh(s, x, v, g, h);                                                             \
+  }                                                                                                 \
+  void eval_multi_UBspline_2d_##T(multi_UBspline_2d_##T *s, double x, double y, TY *v) {            \
+    c_eval_2d_##T(s, x, y, v);                                                                      \
+  }                                                                                                 \
+  void eval_multi_UBspline_2d_##T##_vg(multi_UBspline_2d_##T *s, double x, double y, TY *v, TY *g) { \
+    c_eval_2d_##T##_vg(s, x, y, v, g);                                                              \
+  }                                                                                                 \
+  void eval_multi_UBspline_2d_##T##_vgl(multi_UBspline_2d_##T *s, double x, double y, TY *v, TY *g, \
+                                        TY *l) {                                                    \
+    c_eval_2d_##T##_vgl(s, x, y, v, g, l);                                                          \
+  }                                                                                                 \
+  void eval_multi_UBspline_2d_##T##_vgh(multi_UBspline_2d_##T *s, double x, double y, TY *v, TY *g, \
+                                        TY *h) {                                                    \
+    c_eval_2d_##T##_vgh(s, x, y, v, g, h);                                                          \
+  }
+ESK_FWD12(s, float)
+ESK_FWD12(d, double)
+ESK_FWD12(c, std::complex<float>)
+ESK_FWD12(z, std::complex<double>)

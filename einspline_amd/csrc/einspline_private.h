@@ -13,18 +13,28 @@ extern "C" {
 
 #define EINSPLINE_HOST_MAGIC 0x45534833u /* "ESH3" */
 
-/* Every 3-D multi-spline returned by create_multi_UBspline_3d_* is the
+/* Every multi-spline returned by create_multi_UBspline_{1,2,3}d_* is the
  * public struct followed by this private tail, so a pointer to the public
- * struct is a pointer to the whole object. */
+ * struct is a pointer to the whole object.  All public structs share the
+ * prefix {spcode, tcode, coefs}. */
 typedef struct einspline_host3d {
   union {
     multi_UBspline_3d_s s;
     multi_UBspline_3d_d d;
     multi_UBspline_3d_c c;
     multi_UBspline_3d_z z;
+    multi_UBspline_2d_s s2;
+    multi_UBspline_2d_d d2;
+    multi_UBspline_2d_c c2;
+    multi_UBspline_2d_z z2;
+    multi_UBspline_1d_s s1;
+    multi_UBspline_1d_d d1;
+    multi_UBspline_1d_c c1;
+    multi_UBspline_1d_z z1;
   } pub;
   uint32_t magic;
-  int Nx, Ny, Nz;          /* coefficient-table extent per axis */
+  int dim;                 /* 1, 2 or 3 */
+  int Nx, Ny, Nz;          /* coefficient-table extent per axis (1 past dim) */
   int ncomp;               /* reals per element: 1 real, 2 complex */
   int nsplines;            /* num_splines (its offset differs between the union members) */
   int dirty;               /* host coefs changed since the device mirror was built */

@@ -28,6 +28,13 @@ namespace esk {
 #ifndef ESK_WAVES_PER_EU
 #define ESK_WAVES_PER_EU 4
 #endif
+// ESK_YX: contract y before x.  For each x row i the z-reduced partials of
+// the four y rows are combined with the y basis into per-i partials, which
+// the x basis then folds into the accumulators (fewer FMAs, and no products
+// of two uniform basis values, at the cost of the per-i partials' registers).
+#ifndef ESK_YX
+#define ESK_YX 0
+#endif
 #ifndef ESK_PREFETCH
 #define ESK_PREFETCH 1
 #endif
@@ -35,7 +42,9 @@ namespace esk {
 
 // MODE_QMC: value, lattice-transformed gradient and Laplacian in the layout
 // of the reference's CUDA vgl (src/multi_bspline_cuda_s_impl.h:566-734).
-enum { MODE_V = 0, MODE_VG = 1, MODE_VGL = 2, MODE_VGH = 3, MODE_QMC = 4 };
+// MODE_VGHGH: vgh plus the 27 third derivatives per orbital
+// (src/multi_bspline_eval_std_d_impl.h:868-1049).
+enum { MODE_V = 0, MODE_VG = 1, MODE_VGL = 2, MODE_VGH = 3, MODE_QMC = 4, MODE_VGHGH = 5 };
 
 // Storage tags for 16-bit tables (accumulated in float).
 struct bf16s { uint16_t b; };
@@ -58,6 +67,8 @@ struct Params {
   const void *pos;           // positions, 3 per point (double or float)
   void *out_v, *out_g, *out_h;   // strided bases, or device arrays of per-point pointers
   long ld_v, ld_g, ld_h;     // per-point strides of strided outputs, in reals
+  void *out_gh;              // MODE_VGHGH: third derivatives, 27 per orbital
+  long ld_gh;
   int hpacked;               // vgh: 1 -> 6 unique Hessian entries per orbital, 0 -> full 9
   const void *linv;          // MODE_QMC: 3x3 row-major lattice matrix G (accumulation type)
   const void *sign;          // optional per-point factor applied to every output (or null)
@@ -279,11 +290,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One LDS staging area per (type, size); a kernel instantiates one of them.
-template <class A, int N>
+// LDS staging area for the transposed result stores: dynamic LDS, sized by
+// the launch (StageLds below) only when the store policy stages, so kernels
+// that store directly keep the whole LDS for occupancy.
+template <class A>
 __device__ __forceinline__ A *lds_stage() {
-  __shared__ A buf[N];
-  return buf;
+  extern __shared__ __align__(16) unsigned char esk_stage_lds[];
+  return (A *)esk_stage_lds;
 }
 
 // Store a wavefront's contiguous output region of 64*K reals, where lane l
@@ -334,13 +347,33 @@ struct Occupancy {
   using A = typename acc_of<S>::type;
   static constexpr int V = LB / (int)sizeof(S);
   static constexpr int W = (int)sizeof(A) / 4;              // VGPRs per value
-  static constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7 : 10;
+  static constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7
+                           : MODE == MODE_VGHGH ? 20 : 10;
   static constexpr int NW = MODE == MODE_V ? 12 : MODE == MODE_VG ? 24 : 36;
   // double weights overflow the scalar register file, so half of them are
   // counted as vector registers even for one point per wavefront
-  static constexpr int regs = NACC * V * W + (ESK_PREFETCH + 1) * 4 * V * W + 3 * V * W +
+  // per-x-row y partials of the y-then-x contraction
+  static constexpr int NT = !ESK_YX ? 0 : MODE == MODE_V ? 1 : MODE == MODE_VG ? 3 : MODE == MODE_VGL ? 5
+                          : MODE == MODE_VGHGH ? 10 : 6;
+  static constexpr int regs = NACC * V * W + (ESK_PREFETCH + 1) * 4 * V * W + 3 * V * W + NT * V * W +
                               (UNIF ? (W == 2 ? NW : 0) : NW * W) + 24;
   static constexpr int waves = regs <= 116 ? ESK_WAVES_PER_EU : regs <= 160 ? 3 : regs <= 248 ? 2 : 1;
+};
+
+// Staged stores: used by one-point-per-wavefront instantiations with
+// gradients; each wavefront needs kmax*64 reals of LDS (kmax: the widest
+// staged block of one lane, 3V gradients or 9V Hessian entries).
+template <class S, int LB, int MODE, bool UNIF>
+struct StageLds {
+  using A = typename acc_of<S>::type;
+  static constexpr int V = LB / (int)sizeof(S);
+  static constexpr bool on = UNIF && MODE >= MODE_VG && MODE != MODE_QMC && ((3 * V * sizeof(A)) % 8 == 0);
+  static constexpr int kmax = (MODE == MODE_VGH || MODE == MODE_VGHGH ? 9 : 3) * V;
+  static constexpr size_t wave_bytes = on ? (size_t)kmax * 64 * sizeof(A) : 0;
+  // dynamic LDS of a launch of `block` work-items with store policy `store`
+  static size_t bytes(int block, int store) {
+    return store != STORE_DIRECT ? (size_t)(block / 64) * wave_bytes : 0;
+  }
 };
 
 // Evaluate chunk `ch` of point `p` (0 <= p < P.npoints, 0 <= ch < P.chunks).
@@ -379,8 +412,9 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   }
 
   // accumulators
-  constexpr bool HESS = MODE == MODE_VGH || MODE == MODE_QMC;
-  constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7 : 10;
+  constexpr bool HESS = MODE == MODE_VGH || MODE == MODE_QMC || MODE == MODE_VGHGH;
+  constexpr bool THIRD = MODE == MODE_VGHGH;
+  constexpr int NACC = MODE == MODE_V ? 1 : MODE == MODE_VG ? 4 : MODE == MODE_VGL ? 7 : THIRD ? 20 : 10;
   A acc[NACC][V];
 #pragma unroll
   for (int q = 0; q < NACC; ++q)
@@ -403,6 +437,12 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) L::load(rp + k * zs, rb[st][k]);
   }
+#if ESK_YX
+  // y partials per x row: b s0, db s0, d2b s0, b s1, db s1, b s2 (+ d3b s0,
+  // d2b s1, db s2, b s3 for third derivatives)
+  constexpr int NT = MODE == MODE_V ? 1 : MODE == MODE_VG ? 3 : MODE == MODE_VGL ? 5 : THIRD ? 10 : 6;
+  A tq[NT][V];
+#endif
 #pragma unroll
   for (int st = 0; st < 16; ++st) {
     const int i = st >> 2, j = st & 3;
@@ -414,9 +454,13 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
     }
     A (&r)[4][V] = rb[st % (PF + 1)];
 
-    A s0[V], s1[V], s2[V];
+    A s0[V], s1[V], s2[V], s3[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
+      if constexpr (THIRD) {
+        // third-derivative z weights are the constants -1, 3, -3, 1
+        s3[e] = fmaA((A)3, r[1][e] - r[2][e], r[3][e] - r[0][e]);
+      }
       s0[e] = c[0] * r[0][e];
       s0[e] = fmaA(c[1], r[1][e], s0[e]);
       s0[e] = fmaA(c[2], r[2][e], s0[e]);
@@ -434,6 +478,103 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         s2[e] = fmaA(d2c[3], r[3][e], s2[e]);
       }
     }
+#if ESK_YX
+    {
+      // y step: fold row j into the per-i partials (basis values straight from
+      // scalar registers); the first row of each i initialises them
+      constexpr A D3[4] = {(A)-1, (A)3, (A)-3, (A)1};
+      A w[NT];
+      int q = 0;
+      w[q++] = b[j];
+      if constexpr (MODE >= MODE_VG) { w[q++] = db[j]; }
+      if constexpr (MODE == MODE_VGL) { w[q++] = d2b[j]; }
+      if constexpr (HESS) { w[q++] = d2b[j]; }
+      if constexpr (THIRD) { w[q++] = D3[j]; }
+      // source partial of each y partial: 0 -> s0, 1 -> s1, 2 -> s2, 3 -> s3
+      constexpr int NS0 = MODE == MODE_V ? 1 : MODE == MODE_VG ? 2 : MODE == MODE_VGL ? 3 : THIRD ? 4 : 3;
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+#pragma unroll
+        for (int u = 0; u < NS0; ++u) {           // b s0, db s0, d2b s0, d3b s0
+          if (j == 0) tq[u][e] = w[u] * s0[e];
+          else tq[u][e] = fmaA(w[u], s0[e], tq[u][e]);
+        }
+        if constexpr (MODE >= MODE_VG) {
+          // b s1, (db s1), (d2b s1)
+          constexpr int NS1 = MODE == MODE_VG ? 1 : MODE == MODE_VGL ? 1 : THIRD ? 3 : 2;
+#pragma unroll
+          for (int u = 0; u < NS1; ++u) {
+            const A wu = u == 0 ? b[j] : u == 1 ? db[j] : d2b[j];
+            if (j == 0) tq[NS0 + u][e] = wu * s1[e];
+            else tq[NS0 + u][e] = fmaA(wu, s1[e], tq[NS0 + u][e]);
+          }
+          if constexpr (MODE >= MODE_VGL) {
+            // b s2, (db s2)
+            constexpr int NS2 = THIRD ? 2 : 1;
+#pragma unroll
+            for (int u = 0; u < NS2; ++u) {
+              const A wu = u == 0 ? b[j] : db[j];
+              if (j == 0) tq[NS0 + NS1 + u][e] = wu * s2[e];
+              else tq[NS0 + NS1 + u][e] = fmaA(wu, s2[e], tq[NS0 + NS1 + u][e]);
+            }
+            if constexpr (THIRD) {
+              if (j == 0) tq[NS0 + NS1 + NS2][e] = b[j] * s3[e];
+              else tq[NS0 + NS1 + NS2][e] = fmaA(b[j], s3[e], tq[NS0 + NS1 + NS2][e]);
+            }
+          }
+        }
+      }
+    }
+    if (j == 3) {
+      // x step: fold the per-i partials into the accumulators.  Partial
+      // index: V/VG/VGL/VGH: 0 b s0, 1 db s0, 2 d2b s0 (VGL, VGH), then s1
+      // terms, then s2 terms; VGHGH: 0 b s0, 1 db s0, 2 d2b s0, 3 d3b s0,
+      // 4 b s1, 5 db s1, 6 d2b s1, 7 b s2, 8 db s2, 9 b s3.
+      constexpr A D3[4] = {(A)-1, (A)3, (A)-3, (A)1};
+      constexpr int BS0 = 0, DBS0 = 1, D2BS0 = 2;
+      constexpr int BS1 = MODE == MODE_VG ? 2 : THIRD ? 4 : 3;
+      constexpr int DBS1 = THIRD ? 5 : 4;
+      constexpr int BS2 = MODE == MODE_VGL ? 4 : THIRD ? 7 : 5;
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        acc[0][e] = fmaA(a[i], tq[BS0][e], acc[0][e]);                    // v
+        if constexpr (MODE >= MODE_VG) {
+          acc[1][e] = fmaA(da[i], tq[BS0][e], acc[1][e]);                 // d/dx
+          acc[2][e] = fmaA(a[i], tq[DBS0][e], acc[2][e]);                 // d/dy
+          acc[3][e] = fmaA(a[i], tq[BS1][e], acc[3][e]);                  // d/dz
+        }
+        if constexpr (MODE == MODE_VGL) {
+          acc[4][e] = fmaA(d2a[i], tq[BS0][e], acc[4][e]);                // d2/dx2
+          acc[5][e] = fmaA(a[i], tq[D2BS0][e], acc[5][e]);                // d2/dy2
+          acc[6][e] = fmaA(a[i], tq[BS2][e], acc[6][e]);                  // d2/dz2
+        }
+        if constexpr (HESS) {
+          acc[4][e] = fmaA(d2a[i], tq[BS0][e], acc[4][e]);                // xx
+          acc[5][e] = fmaA(da[i], tq[DBS0][e], acc[5][e]);                // xy
+          acc[6][e] = fmaA(da[i], tq[BS1][e], acc[6][e]);                 // xz
+          acc[7][e] = fmaA(a[i], tq[D2BS0][e], acc[7][e]);                // yy
+          acc[8][e] = fmaA(a[i], tq[DBS1][e], acc[8][e]);                 // yz
+          acc[9][e] = fmaA(a[i], tq[BS2][e], acc[9][e]);                  // zz
+        }
+        if constexpr (THIRD) {
+          acc[10][e] = fmaA(D3[i], tq[BS0][e], acc[10][e]);               // xxx
+          acc[11][e] = fmaA(d2a[i], tq[DBS0][e], acc[11][e]);             // xxy
+          acc[12][e] = fmaA(d2a[i], tq[BS1][e], acc[12][e]);              // xxz
+          acc[13][e] = fmaA(da[i], tq[D2BS0][e], acc[13][e]);             // xyy
+          acc[14][e] = fmaA(da[i], tq[DBS1][e], acc[14][e]);              // xyz
+          acc[15][e] = fmaA(da[i], tq[BS2][e], acc[15][e]);               // xzz
+          acc[16][e] = fmaA(a[i], tq[3][e], acc[16][e]);                  // yyy
+          acc[17][e] = fmaA(a[i], tq[6][e], acc[17][e]);                  // yyz
+          acc[18][e] = fmaA(a[i], tq[8][e], acc[18][e]);                  // yzz
+          acc[19][e] = fmaA(a[i], tq[9][e], acc[19][e]);                  // zzz
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NT; ++q)
+#pragma unroll
+      for (int e = 0; e < V; ++e) pin(tq[q][e]);
+#else
     const A ab = a[i] * b[j];
     if constexpr (MODE == MODE_V) {
 #pragma unroll
@@ -468,7 +609,28 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           acc[9][e] = fmaA(ab, s2[e], acc[9][e]);    // zz
         }
       }
+      if constexpr (THIRD) {
+        // d3a[i], d3b[j] are the constants -1, 3, -3, 1
+        constexpr A D3[4] = {(A)-1, (A)3, (A)-3, (A)1};
+        const A d3ab = D3[i] * b[j], d2adb = d2a[i] * db[j], d2ab = d2a[i] * b[j];
+        const A dad2b = da[i] * d2b[j], dadb = da[i] * db[j], dab = da[i] * b[j];
+        const A ad3b = a[i] * D3[j], ad2b = a[i] * d2b[j], adb = a[i] * db[j];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          acc[10][e] = fmaA(d3ab, s0[e], acc[10][e]);   // xxx
+          acc[11][e] = fmaA(d2adb, s0[e], acc[11][e]);  // xxy
+          acc[12][e] = fmaA(d2ab, s1[e], acc[12][e]);   // xxz
+          acc[13][e] = fmaA(dad2b, s0[e], acc[13][e]);  // xyy
+          acc[14][e] = fmaA(dadb, s1[e], acc[14][e]);   // xyz
+          acc[15][e] = fmaA(dab, s2[e], acc[15][e]);    // xzz
+          acc[16][e] = fmaA(ad3b, s0[e], acc[16][e]);   // yyy
+          acc[17][e] = fmaA(ad2b, s1[e], acc[17][e]);   // yyz
+          acc[18][e] = fmaA(adb, s2[e], acc[18][e]);    // yzz
+          acc[19][e] = fmaA(ab, s3[e], acc[19][e]);     // zzz
+        }
+      }
     }
+#endif
     // Materialise the accumulators here: without this the optimiser sinks
     // the reductions past later loads and keeps every loaded row alive.
 #pragma unroll
@@ -502,6 +664,15 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
       acc[4][e] *= sxx; acc[5][e] *= sxy; acc[6][e] *= sxz;
       acc[7][e] *= syy; acc[8][e] *= syz; acc[9][e] *= szz;
     }
+  }
+  if constexpr (THIRD) {
+    const A f[10] = {dxi * dxi * dxi, dxi * dxi * dyi, dxi * dxi * dzi, dxi * dyi * dyi,
+                     dxi * dyi * dzi, dxi * dzi * dzi, dyi * dyi * dyi, dyi * dyi * dzi,
+                     dyi * dzi * dzi, dzi * dzi * dzi};
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+#pragma unroll
+      for (int e = 0; e < V; ++e) acc[10 + q][e] *= f[q];
   }
 
   // ---- outputs ----
@@ -562,15 +733,15 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   // One point per wavefront, 64 full chunks and aligned outputs: the
   // wavefront's gradient and Hessian blocks are each one contiguous region,
   // written through LDS with fully coalesced stores.
-  constexpr bool STAGE = UNIF && MODE >= MODE_VG && MODE != MODE_QMC && ((3 * V * sizeof(A)) % 8 == 0);
+  constexpr bool STAGE = StageLds<S, LB, MODE, UNIF>::on;
   const int lane = threadIdx.x & 63;
   const int wave_e0 = (ch - lane) * V;   // first real of the wavefront (UNIF)
   bool staged = false;
   A *lds = nullptr;
   if constexpr (STAGE) {
     staged = vec && P.store != STORE_DIRECT && (wave_e0 + 64 * V <= P.nreal);
-    constexpr int KMAX = (MODE == MODE_VGH ? 9 : 3) * V;
-    lds = lds_stage<A, 4 * KMAX * 64>() + (threadIdx.x >> 6) * (KMAX * 64);
+    constexpr int KMAX = StageLds<S, LB, MODE, UNIF>::kmax;
+    lds = lds_stage<A>() + (threadIdx.x >> 6) * (KMAX * 64);
   }
 
   // values (and the Laplacian) keep the real index layout
@@ -605,7 +776,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
       else store_partial<A, 3 * V>(og + 3 * e0, r, 3 * m);
     }
   }
-  if constexpr (MODE == MODE_VGH) {
+  if constexpr (MODE == MODE_VGH || THIRD) {
     // acc index of each Hessian entry, row-major 3x3 and packed upper triangle
     constexpr int full9[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
     constexpr int pack6[6] = {4, 5, 6, 7, 8, 9};
@@ -642,6 +813,25 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
       }
     }
+  }
+  if constexpr (THIRD) {
+    // full symmetric 3x3x3 tensor per orbital: slot (a,b,c) holds the entry of
+    // the sorted triple (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz -> acc 10..19)
+    constexpr int sym27[27] = {10, 11, 12, 11, 13, 14, 12, 14, 15,
+                               11, 13, 14, 13, 16, 17, 14, 17, 18,
+                               12, 14, 15, 14, 17, 18, 15, 18, 19};
+    A *ogh = (A *)P.out_gh + (long)p * P.ld_gh;
+    A r[27 * V];
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+#pragma unroll
+      for (int h = 0; h < 27; ++h) {
+        const int o = CPLX ? (e >> 1) * 54 + h * 2 + (e & 1) : e * 27 + h;
+        r[o] = acc[sym27[h]][e];
+      }
+    const bool gvec = vec && ((uintptr_t)ogh & 15) == 0;
+    if (full) store_block<A, 27 * V>(ogh + 27 * e0, r, gvec, false);
+    else store_partial<A, 27 * V>(ogh + 27 * e0, r, 27 * m);
   }
 }
 

@@ -21,6 +21,21 @@ hipError_t launch_f32(const Launch &L, const Params &P, hipStream_t s);
 hipError_t launch_f64(const Launch &L, const Params &P, hipStream_t s);
 hipError_t launch_bf16(const Launch &L, const Params &P, hipStream_t s);
 hipError_t launch_f16(const Launch &L, const Params &P, hipStream_t s);
+// 1-D and 2-D tables (FP32 or FP64 storage, 16-byte lanes), launch12.hip.
+hipError_t launch12(const Launch &L, const Params &P, int dim, bool f64, hipStream_t s);
+
+// Launch with `lds` bytes of dynamic LDS; more than 64 KB needs the
+// kernel's limit raised (set on every such launch: it is per kernel).
+template <class K>
+hipError_t launch_lds(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, const Params &P) {
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kernel, grid, block, lds, s, P);
+  return hipGetLastError();
+}
 
 #ifdef ESK_INSTANTIATE
 // Instantiation helpers, used only by launch_<type>.hip.
@@ -28,22 +43,17 @@ template <class S, int LB, int MODE, bool CPLX, bool UNIF>
 hipError_t go_api(const Launch &L, const Params &P, hipStream_t s) {
   const long total = (long)P.npoints * P.chunks;
   dim3 block(L.block), grid((unsigned)((total + L.block - 1) / L.block));
+  const size_t lds = StageLds<S, LB, MODE, UNIF>::bytes(L.block, P.store);
   if (L.api == API_STRIDED) {
-    if constexpr (MODE != MODE_QMC) {
-      hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, double, false>), grid, block, 0, s, P);
-      return hipGetLastError();
-    }
+    if constexpr (MODE != MODE_QMC)
+      return launch_lds(eval3d<S, LB, MODE, CPLX, UNIF, double, false>, grid, block, lds, s, P);
     return hipErrorInvalidValue;
   }
   if constexpr (LB == 16 && (MODE == MODE_V || MODE == MODE_VGH || MODE == MODE_QMC)) {
-    if (L.api == API_PTRS_F32POS) {
-      hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, float, true>), grid, block, 0, s, P);
-      return hipGetLastError();
-    }
-    if (L.api == API_PTRS_F64POS) {
-      hipLaunchKernelGGL((eval3d<S, LB, MODE, CPLX, UNIF, double, true>), grid, block, 0, s, P);
-      return hipGetLastError();
-    }
+    if (L.api == API_PTRS_F32POS)
+      return launch_lds(eval3d<S, LB, MODE, CPLX, UNIF, float, true>, grid, block, lds, s, P);
+    if (L.api == API_PTRS_F64POS)
+      return launch_lds(eval3d<S, LB, MODE, CPLX, UNIF, double, true>, grid, block, lds, s, P);
   }
   return hipErrorInvalidValue;
 }
@@ -64,6 +74,9 @@ hipError_t go_mode(const Launch &L, const Params &P, hipStream_t s) {
     case MODE_VGH: return go_unif<S, LB, MODE_VGH, CPLX>(L, P, s);
     case MODE_QMC:
       if constexpr (LB == 16 && sizeof(S) >= 4) return go_unif<S, LB, MODE_QMC, CPLX>(L, P, s);
+      break;
+    case MODE_VGHGH:   // 20 accumulators per real: at most 16 bytes and 4 reals per lane
+      if constexpr (LB <= 16 && LB / sizeof(S) <= 4) return go_unif<S, LB, MODE_VGHGH, CPLX>(L, P, s);
       break;
   }
   return hipErrorInvalidValue;

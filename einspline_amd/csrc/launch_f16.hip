@@ -8,7 +8,6 @@ hipError_t launch_f16(const Launch &L, const Params &P, hipStream_t s) {
   switch (L.lane_bytes) {
     case 8: return go_cplx<f16s, 8>(L, P, s);
     case 16: return go_cplx<f16s, 16>(L, P, s);
-    case 32: return go_cplx<f16s, 32>(L, P, s);
   }
   return hipErrorInvalidValue;
 }

@@ -1,9 +1,10 @@
-/* multi_create.c -- host construction of 3-D uniform multi-splines.
+/* multi_create.c -- host construction of uniform multi-splines.
  *
- * Implements create_multi_UBspline_3d_{s,d,c,z}, set_multi_UBspline_3d_*
- * (+ block variants) and destroy_Bspline with the behaviour of
- * src/multi_bspline_create.c:251-373 (s), :576-732 (c), :941-1073 (d),
- * :1317-1477 (z), :1480-1485 (destroy).
+ * Implements create_multi_UBspline_{1,2,3}d_{s,d,c,z},
+ * set_multi_UBspline_{1,2,3}d_* (+ 3-D block variants, 1-D _BC variants)
+ * and destroy_Bspline with the behaviour of src/multi_bspline_create.c:
+ * 88-373 (s), :389-732 (c), :767-1073 (d), :1091-1477 (z), :1480-1485
+ * (destroy).
  *
  * Interpolation conditions (one axis, M data points d_0..d_{M-1}):
  *   every data point:  c_{i}/6 + 2 c_{i+1}/3 + c_{i+2}/6 = d_i
@@ -12,8 +13,9 @@
  *   DERIV1 / FLAT:     (c_2 - c_0) * delta_inv / 2          = value  (M+2 coefs)
  *   DERIV2 / NATURAL:  (c_0 - 2 c_1 + c_2) * delta_inv^2    = value
  * (src/bspline_create.c:1115-1198 sets up these rows; FLAT and NATURAL
- * force value = 0.)  The 3-D table is the tensor product: the 1-D problem is
- * solved along x, then y, then z.
+ * force value = 0.)  A 2-D or 3-D table is the tensor product: the 1-D
+ * problem is solved along x, then y, then z.  A 1-D or 2-D table is held as
+ * a 3-D one whose missing axes have extent 1 and are not solved.
  *
  * The solve is organised differently from the reference (which eliminates
  * a banded matrix per line, src/bspline_create.c:97-145, :958-1055): each
@@ -253,19 +255,21 @@ static void fill_grid(Ugrid *g, int N) {
   g->delta_inv = 1.0 / g->delta;
 }
 
-/* Shared part of the four create functions.  `pub` points at the public
- * struct inside the returned object; the caller fills BCs and the typed
- * coefs pointer from `table`. */
-static einspline_host3d *create_common(const char *fn, Ugrid *xg, Ugrid *yg, Ugrid *zg,
+/* Shared part of the create functions.  `pub` points at the public struct
+ * inside the returned object; the caller fills BCs and the typed coefs
+ * pointer from `table`.  Axes past `dim` pass NULL grids (extent 1). */
+static einspline_host3d *create_common(const char *fn, int dim, Ugrid *xg, Ugrid *yg, Ugrid *zg,
                                        bc_code xl, bc_code yl, bc_code zl, int num_splines,
                                        size_t elt, int ncomp, void **table) {
-  int Nx, Ny, Nz;
-  if (num_splines < 1 || axis_extent(xg, xl, &Nx) || axis_extent(yg, yl, &Ny) ||
-      axis_extent(zg, zl, &Nz)) {
+  int Nx = 1, Ny = 1, Nz = 1;
+  if (num_splines < 1 || axis_extent(xg, xl, &Nx) || (dim >= 2 && axis_extent(yg, yl, &Ny)) ||
+      (dim >= 3 && axis_extent(zg, zl, &Nz))) {
     fprintf(stderr, "%s: invalid grid (num >= 2, end != start) or num_splines < 1\n", fn);
     return NULL;
   }
-  fill_grid(xg, Nx); fill_grid(yg, Ny); fill_grid(zg, Nz);
+  fill_grid(xg, Nx);
+  if (dim >= 2) fill_grid(yg, Ny);
+  if (dim >= 3) fill_grid(zg, Nz);
   einspline_host3d *h = calloc(1, sizeof(*h));
   if (!h) {
     fprintf(stderr, "Out of memory allocating spline in %s.\n", fn);
@@ -280,6 +284,7 @@ static einspline_host3d *create_common(const char *fn, Ugrid *xg, Ugrid *yg, Ugr
   memset(t, 0, bytes);
   *table = t;
   h->magic = EINSPLINE_HOST_MAGIC;
+  h->dim = dim;
   h->Nx = Nx; h->Ny = Ny; h->Nz = Nz;
   h->ncomp = ncomp;
   h->nsplines = num_splines;
@@ -303,7 +308,7 @@ multi_UBspline_3d_s *create_multi_UBspline_3d_s(Ugrid x_grid, Ugrid y_grid, Ugri
                                                 BCtype_s xBC, BCtype_s yBC, BCtype_s zBC,
                                                 int num_splines) {
   void *t;
-  einspline_host3d *h = create_common("create_multi_UBspline_3d_s", &x_grid, &y_grid, &z_grid,
+  einspline_host3d *h = create_common("create_multi_UBspline_3d_s", 3, &x_grid, &y_grid, &z_grid,
                                       xBC.lCode, yBC.lCode, zBC.lCode, num_splines,
                                       sizeof(float), 1, &t);
   if (!h) return NULL;
@@ -317,7 +322,7 @@ multi_UBspline_3d_d *create_multi_UBspline_3d_d(Ugrid x_grid, Ugrid y_grid, Ugri
                                                 BCtype_d xBC, BCtype_d yBC, BCtype_d zBC,
                                                 int num_splines) {
   void *t;
-  einspline_host3d *h = create_common("create_multi_UBspline_3d_d", &x_grid, &y_grid, &z_grid,
+  einspline_host3d *h = create_common("create_multi_UBspline_3d_d", 3, &x_grid, &y_grid, &z_grid,
                                       xBC.lCode, yBC.lCode, zBC.lCode, num_splines,
                                       sizeof(double), 1, &t);
   if (!h) return NULL;
@@ -331,7 +336,7 @@ multi_UBspline_3d_c *create_multi_UBspline_3d_c(Ugrid x_grid, Ugrid y_grid, Ugri
                                                 BCtype_c xBC, BCtype_c yBC, BCtype_c zBC,
                                                 int num_splines) {
   void *t;
-  einspline_host3d *h = create_common("create_multi_UBspline_3d_c", &x_grid, &y_grid, &z_grid,
+  einspline_host3d *h = create_common("create_multi_UBspline_3d_c", 3, &x_grid, &y_grid, &z_grid,
                                       xBC.lCode, yBC.lCode, zBC.lCode, num_splines,
                                       2 * sizeof(float), 2, &t);
   if (!h) return NULL;
@@ -350,7 +355,7 @@ multi_UBspline_3d_z *create_multi_UBspline_3d_z(Ugrid x_grid, Ugrid y_grid, Ugri
                                                 BCtype_z xBC, BCtype_z yBC, BCtype_z zBC,
                                                 int num_splines) {
   void *t;
-  einspline_host3d *h = create_common("create_multi_UBspline_3d_z", &x_grid, &y_grid, &z_grid,
+  einspline_host3d *h = create_common("create_multi_UBspline_3d_z", 3, &x_grid, &y_grid, &z_grid,
                                       xBC.lCode, yBC.lCode, zBC.lCode, num_splines,
                                       2 * sizeof(double), 2, &t);
   if (!h) return NULL;
@@ -365,6 +370,60 @@ multi_UBspline_3d_z *create_multi_UBspline_3d_z(Ugrid x_grid, Ugrid y_grid, Ugri
   return s;
 }
 
+/* 1-D and 2-D: the same object with Ny (and Nz) = 1. */
+#define CREATE_1D(T, TY, BCT, TC, ELT, NC)                                                     \
+  multi_UBspline_1d_##T *create_multi_UBspline_1d_##T(Ugrid x_grid, BCT xBC, int num_splines) { \
+    void *t;                                                                                  \
+    einspline_host3d *h = create_common("create_multi_UBspline_1d_" #T, 1, &x_grid, NULL, NULL, \
+                                        xBC.lCode, PERIODIC, PERIODIC, num_splines, ELT, NC, &t); \
+    if (!h) return NULL;                                                                      \
+    multi_UBspline_1d_##T *s = &h->pub.T##1;                                                  \
+    s->spcode = MULTI_U1D;                                                                    \
+    s->tcode = TC;                                                                            \
+    s->x_grid = x_grid;                                                                       \
+    s->xBC = xBC;                                                                             \
+    s->num_splines = num_splines;                                                             \
+    s->x_stride = num_splines;                                                                \
+    s->coefs = (TY *)t;                                                                       \
+    return s;                                                                                 \
+  }
+CREATE_1D(s, float, BCtype_s, SINGLE_REAL, sizeof(float), 1)
+CREATE_1D(d, double, BCtype_d, DOUBLE_REAL, sizeof(double), 1)
+CREATE_1D(c, complex_float, BCtype_c, SINGLE_COMPLEX, 2 * sizeof(float), 2)
+CREATE_1D(z, complex_double, BCtype_z, DOUBLE_COMPLEX, 2 * sizeof(double), 2)
+
+#define CREATE_2D(T, TY, BCT, TC, ELT, NC, LAPL2)                                              \
+  multi_UBspline_2d_##T *create_multi_UBspline_2d_##T(Ugrid x_grid, Ugrid y_grid, BCT xBC,     \
+                                                      BCT yBC, int num_splines) {             \
+    void *t;                                                                                  \
+    einspline_host3d *h = create_common("create_multi_UBspline_2d_" #T, 2, &x_grid, &y_grid,   \
+                                        NULL, xBC.lCode, yBC.lCode, PERIODIC, num_splines,    \
+                                        ELT, NC, &t);                                         \
+    if (!h) return NULL;                                                                      \
+    multi_UBspline_2d_##T *s = &h->pub.T##2;                                                  \
+    s->spcode = MULTI_U2D;                                                                    \
+    s->tcode = TC;                                                                            \
+    s->x_grid = x_grid; s->y_grid = y_grid;                                                   \
+    s->xBC = xBC; s->yBC = yBC;                                                               \
+    s->num_splines = num_splines;                                                             \
+    s->y_stride = num_splines;                                                                \
+    s->x_stride = (intptr_t)h->Ny * num_splines;                                              \
+    s->coefs = (TY *)t;                                                                       \
+    LAPL2                                                                                     \
+    return s;                                                                                 \
+  }
+#define NO_LAPL2
+#define WITH_LAPL2(TY)                                                                        \
+  s->lapl2 = calloc(2 * (size_t)num_splines, sizeof(TY));                                    \
+  if (!s->lapl2) {                                                                            \
+    fprintf(stderr, "Out of memory allocating spline coefficients in create_multi_UBspline_2d.\n"); \
+    abort();                                                                                  \
+  }
+CREATE_2D(s, float, BCtype_s, SINGLE_REAL, sizeof(float), 1, NO_LAPL2)
+CREATE_2D(d, double, BCtype_d, DOUBLE_REAL, sizeof(double), 1, NO_LAPL2)
+CREATE_2D(c, complex_float, BCtype_c, SINGLE_COMPLEX, 2 * sizeof(float), 2, WITH_LAPL2(complex_float))
+CREATE_2D(z, complex_double, BCtype_z, DOUBLE_COMPLEX, 2 * sizeof(double), 2, WITH_LAPL2(complex_double))
+
 /* ------------------------------------------------------------------------ */
 /* set                                                                       */
 
@@ -374,21 +433,24 @@ typedef struct {
 } axis_bc;
 
 /* Solve `count` orbitals starting at `first`.  src holds count arrays of
- * M^3 elements, each element `ncomp` reals of type given by src_is_float.
- * The result is written into the table (float or double reals). */
+ * Mx*My*Mz elements (My = Mz = 1 past the table's dim), each element
+ * `ncomp` reals of type given by src_is_float.  The result is written into
+ * the table (float or double reals). */
 static void solve_block(einspline_host3d *h, const Ugrid *grids, const axis_bc *bcs,
                         int first, int count, const void *src, int src_is_float,
                         void *table, int table_is_float, const char *fn) {
-  const int ncomp = h->ncomp;
+  const int ncomp = h->ncomp, dim = h->dim;
   const int Nx = h->Nx, Ny = h->Ny, Nz = h->Nz;
-  const int Mx = grids[0].num, My = grids[1].num, Mz = grids[2].num;
+  const int Mx = grids[0].num, My = dim >= 2 ? grids[1].num : 1, Mz = dim >= 3 ? grids[2].num : 1;
+  const int oy = dim >= 2, oz = dim >= 3;     /* halo offset of the first data node */
   const int N = h->nsplines;
   if (first < 0 || count < 1 || first + count > N) {
     fprintf(stderr, "%s: spline index out of range\n", fn);
     return;
   }
   axis_op ops[3];
-  for (int a = 0; a < 3; ++a) {
+  memset(ops, 0, sizeof ops);
+  for (int a = 0; a < dim; ++a) {
     int rc = axis_init(&ops[a], &grids[a], bcs[a].l, bcs[a].r);
     if (rc == -2) {
       fprintf(stderr, "Out of memory in %s.\n", fn);
@@ -415,12 +477,12 @@ static void solve_block(einspline_host3d *h, const Ugrid *grids, const axis_bc *
     const int cnt = (count - o0) < (int)per ? (count - o0) : (int)per;
     const ptrdiff_t L = (ptrdiff_t)cnt * ncomp;
     memset(G, 0, sizeof(double) * vol * L);
-    /* data node (i,j,k) -> element (i+1, j+1, k+1) */
+    /* data node (i,j,k) -> element (i+1, j+1, k+1) on solved axes */
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < Mx; ++i)
       for (int j = 0; j < My; ++j)
         for (int k = 0; k < Mz; ++k) {
-          double *dst = G + ((((size_t)(i + 1) * Ny + (j + 1)) * Nz + (k + 1)) * L);
+          double *dst = G + ((((size_t)(i + 1) * Ny + (j + oy)) * Nz + (k + oz)) * L);
           const size_t node = ((size_t)i * My + j) * Mz + k;
           for (int o = 0; o < cnt; ++o)
             for (int c = 0; c < ncomp; ++c) {
@@ -444,14 +506,18 @@ static void solve_block(einspline_host3d *h, const Ugrid *grids, const axis_bc *
       }
     }
     /* y sweep, per x plane */
+    if (dim >= 2) {
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < Nx; ++i)
-      axis_apply(&ops[1], G + (size_t)i * Ny * Nz * L, (ptrdiff_t)Nz * L, (ptrdiff_t)Nz * L,
-                 bcs[1].lv, bcs[1].rv, ncomp);
+      for (int i = 0; i < Nx; ++i)
+        axis_apply(&ops[1], G + (size_t)i * Ny * Nz * L, (ptrdiff_t)Nz * L, (ptrdiff_t)Nz * L,
+                   bcs[1].lv, bcs[1].rv, ncomp);
+    }
     /* z sweep, per (x, y) line */
+    if (dim >= 3) {
 #pragma omp parallel for schedule(static)
-    for (int ij = 0; ij < Nx * Ny; ++ij)
-      axis_apply(&ops[2], G + (size_t)ij * Nz * L, L, L, bcs[2].lv, bcs[2].rv, ncomp);
+      for (int ij = 0; ij < Nx * Ny; ++ij)
+        axis_apply(&ops[2], G + (size_t)ij * Nz * L, L, L, bcs[2].lv, bcs[2].rv, ncomp);
+    }
     /* scatter into the table column(s) */
     const size_t row = (size_t)N * ncomp;   /* reals per table row */
 #pragma omp parallel for schedule(static)
@@ -468,7 +534,7 @@ static void solve_block(einspline_host3d *h, const Ugrid *grids, const axis_bc *
     }
   }
   free(G);
-  for (int a = 0; a < 3; ++a) axis_free(&ops[a]);
+  for (int a = 0; a < dim; ++a) axis_free(&ops[a]);
   pthread_mutex_lock(&h->lock);
   h->dirty = 1;
   pthread_mutex_unlock(&h->lock);
@@ -494,9 +560,16 @@ static void bcs_cplx(axis_bc *b, bc_code l, bc_code r, double lr, double li, dou
     return;                                                                   \
   }
 
+#define CHECK_DIM(H, D, FN)                                                   \
+  if ((H)->dim != (D)) {                                                      \
+    fprintf(stderr, "%s: spline has %d dimensions\n", FN, (H)->dim);          \
+    return;                                                                   \
+  }
+
 void set_multi_UBspline_3d_s_block(multi_UBspline_3d_s *s, int first, int count,
                                    const float *data) {
   CHECK_OBJ(h, s, "set_multi_UBspline_3d_s")
+  CHECK_DIM(h, 3, "set_multi_UBspline_3d_s")
   Ugrid g[3] = {s->x_grid, s->y_grid, s->z_grid};
   axis_bc b[3];
   bcs_real(&b[0], s->xBC.lCode, s->xBC.rCode, s->xBC.lVal, s->xBC.rVal);
@@ -508,6 +581,7 @@ void set_multi_UBspline_3d_s_block(multi_UBspline_3d_s *s, int first, int count,
 void set_multi_UBspline_3d_d_block(multi_UBspline_3d_d *s, int first, int count,
                                    const double *data) {
   CHECK_OBJ(h, s, "set_multi_UBspline_3d_d")
+  CHECK_DIM(h, 3, "set_multi_UBspline_3d_d")
   Ugrid g[3] = {s->x_grid, s->y_grid, s->z_grid};
   axis_bc b[3];
   bcs_real(&b[0], s->xBC.lCode, s->xBC.rCode, s->xBC.lVal, s->xBC.rVal);
@@ -519,6 +593,7 @@ void set_multi_UBspline_3d_d_block(multi_UBspline_3d_d *s, int first, int count,
 void set_multi_UBspline_3d_c_block(multi_UBspline_3d_c *s, int first, int count,
                                    const complex_float *data) {
   CHECK_OBJ(h, s, "set_multi_UBspline_3d_c")
+  CHECK_DIM(h, 3, "set_multi_UBspline_3d_c")
   Ugrid g[3] = {s->x_grid, s->y_grid, s->z_grid};
   axis_bc b[3];
   bcs_cplx(&b[0], s->xBC.lCode, s->xBC.rCode, s->xBC.lVal_r, s->xBC.lVal_i, s->xBC.rVal_r, s->xBC.rVal_i);
@@ -530,6 +605,7 @@ void set_multi_UBspline_3d_c_block(multi_UBspline_3d_c *s, int first, int count,
 void set_multi_UBspline_3d_z_block(multi_UBspline_3d_z *s, int first, int count,
                                    const complex_double *data) {
   CHECK_OBJ(h, s, "set_multi_UBspline_3d_z")
+  CHECK_DIM(h, 3, "set_multi_UBspline_3d_z")
   Ugrid g[3] = {s->x_grid, s->y_grid, s->z_grid};
   axis_bc b[3];
   bcs_cplx(&b[0], s->xBC.lCode, s->xBC.rCode, s->xBC.lVal_r, s->xBC.lVal_i, s->xBC.rVal_r, s->xBC.rVal_i);
@@ -551,6 +627,104 @@ void set_multi_UBspline_3d_z(multi_UBspline_3d_z *s, int n, complex_double *data
   set_multi_UBspline_3d_z_block(s, n, 1, data);
 }
 
+/* 1-D and 2-D sets. */
+static int same_extent_class(bc_code a, bc_code b) {
+  const int pa = (a == PERIODIC || a == ANTIPERIODIC), pb = (b == PERIODIC || b == ANTIPERIODIC);
+  return pa == pb;
+}
+
+static void set1d_real(multi_UBspline_1d_d *s_d, multi_UBspline_1d_s *s_s, int n, const void *data,
+                       bc_code l, bc_code r, double lv, double rv, const char *fn) {
+  void *sp = s_d ? (void *)s_d : (void *)s_s;
+  CHECK_OBJ(h, sp, fn)
+  CHECK_DIM(h, 1, fn)
+  const Ugrid g0 = s_d ? s_d->x_grid : s_s->x_grid;
+  const bc_code tl = s_d ? s_d->xBC.lCode : s_s->xBC.lCode;
+  if (!same_extent_class(l, tl)) {
+    fprintf(stderr, "%s: boundary condition changes the table extent\n", fn);
+    return;
+  }
+  Ugrid g[3] = {g0, g0, g0};
+  axis_bc b[3];
+  bcs_real(&b[0], l, r, lv, rv);
+  b[1] = b[2] = b[0];
+  solve_block(h, g, b, n, 1, data, s_s != NULL, s_d ? (void *)s_d->coefs : (void *)s_s->coefs,
+              s_s != NULL, fn);
+}
+
+static void set1d_cplx(multi_UBspline_1d_z *s_z, multi_UBspline_1d_c *s_c, int n, const void *data,
+                       bc_code l, bc_code r, double lr, double li, double rr, double ri,
+                       const char *fn) {
+  void *sp = s_z ? (void *)s_z : (void *)s_c;
+  CHECK_OBJ(h, sp, fn)
+  CHECK_DIM(h, 1, fn)
+  const Ugrid g0 = s_z ? s_z->x_grid : s_c->x_grid;
+  const bc_code tl = s_z ? s_z->xBC.lCode : s_c->xBC.lCode;
+  if (!same_extent_class(l, tl)) {
+    fprintf(stderr, "%s: boundary condition changes the table extent\n", fn);
+    return;
+  }
+  Ugrid g[3] = {g0, g0, g0};
+  axis_bc b[3];
+  bcs_cplx(&b[0], l, r, lr, li, rr, ri);
+  b[1] = b[2] = b[0];
+  solve_block(h, g, b, n, 1, data, s_c != NULL, s_z ? (void *)s_z->coefs : (void *)s_c->coefs,
+              s_c != NULL, fn);
+}
+
+void set_multi_UBspline_1d_s(multi_UBspline_1d_s *s, int n, float *data) {
+  set1d_real(NULL, s, n, data, s->xBC.lCode, s->xBC.rCode, s->xBC.lVal, s->xBC.rVal,
+             "set_multi_UBspline_1d_s");
+}
+void set_multi_UBspline_1d_d(multi_UBspline_1d_d *s, int n, double *data) {
+  set1d_real(s, NULL, n, data, s->xBC.lCode, s->xBC.rCode, s->xBC.lVal, s->xBC.rVal,
+             "set_multi_UBspline_1d_d");
+}
+void set_multi_UBspline_1d_d_BC(multi_UBspline_1d_d *s, int n, double *data, BCtype_d xBC) {
+  set1d_real(s, NULL, n, data, xBC.lCode, xBC.rCode, xBC.lVal, xBC.rVal, "set_multi_UBspline_1d_d_BC");
+}
+void set_multi_UBspline_1d_c(multi_UBspline_1d_c *s, int n, complex_float *data) {
+  set1d_cplx(NULL, s, n, data, s->xBC.lCode, s->xBC.rCode, s->xBC.lVal_r, s->xBC.lVal_i,
+             s->xBC.rVal_r, s->xBC.rVal_i, "set_multi_UBspline_1d_c");
+}
+void set_multi_UBspline_1d_z(multi_UBspline_1d_z *s, int n, complex_double *data) {
+  set1d_cplx(s, NULL, n, data, s->xBC.lCode, s->xBC.rCode, s->xBC.lVal_r, s->xBC.lVal_i,
+             s->xBC.rVal_r, s->xBC.rVal_i, "set_multi_UBspline_1d_z");
+}
+void set_multi_UBspline_1d_z_BC(multi_UBspline_1d_z *s, int n, complex_double *data, BCtype_z xBC) {
+  set1d_cplx(s, NULL, n, data, xBC.lCode, xBC.rCode, xBC.lVal_r, xBC.lVal_i, xBC.rVal_r, xBC.rVal_i,
+             "set_multi_UBspline_1d_z_BC");
+}
+
+#define SET_2D_REAL(T, TY, ISF)                                                                \
+  void set_multi_UBspline_2d_##T(multi_UBspline_2d_##T *s, int n, TY *data) {                  \
+    CHECK_OBJ(h, s, "set_multi_UBspline_2d_" #T)                                                \
+    CHECK_DIM(h, 2, "set_multi_UBspline_2d_" #T)                                                \
+    Ugrid g[3] = {s->x_grid, s->y_grid, s->y_grid};                                            \
+    axis_bc b[3];                                                                              \
+    bcs_real(&b[0], s->xBC.lCode, s->xBC.rCode, s->xBC.lVal, s->xBC.rVal);                     \
+    bcs_real(&b[1], s->yBC.lCode, s->yBC.rCode, s->yBC.lVal, s->yBC.rVal);                     \
+    b[2] = b[1];                                                                               \
+    solve_block(h, g, b, n, 1, data, ISF, s->coefs, ISF, "set_multi_UBspline_2d_" #T);         \
+  }
+#define SET_2D_CPLX(T, TY, ISF)                                                                \
+  void set_multi_UBspline_2d_##T(multi_UBspline_2d_##T *s, int n, TY *data) {                  \
+    CHECK_OBJ(h, s, "set_multi_UBspline_2d_" #T)                                                \
+    CHECK_DIM(h, 2, "set_multi_UBspline_2d_" #T)                                                \
+    Ugrid g[3] = {s->x_grid, s->y_grid, s->y_grid};                                            \
+    axis_bc b[3];                                                                              \
+    bcs_cplx(&b[0], s->xBC.lCode, s->xBC.rCode, s->xBC.lVal_r, s->xBC.lVal_i, s->xBC.rVal_r,   \
+             s->xBC.rVal_i);                                                                   \
+    bcs_cplx(&b[1], s->yBC.lCode, s->yBC.rCode, s->yBC.lVal_r, s->yBC.lVal_i, s->yBC.rVal_r,   \
+             s->yBC.rVal_i);                                                                   \
+    b[2] = b[1];                                                                               \
+    solve_block(h, g, b, n, 1, data, ISF, s->coefs, ISF, "set_multi_UBspline_2d_" #T);         \
+  }
+SET_2D_REAL(s, float, 1)
+SET_2D_REAL(d, double, 0)
+SET_2D_CPLX(c, complex_float, 1)
+SET_2D_CPLX(z, complex_double, 0)
+
 /* ------------------------------------------------------------------------ */
 
 void einspline_hip_invalidate(void *spline) {
@@ -561,8 +735,8 @@ void einspline_hip_invalidate(void *spline) {
   pthread_mutex_unlock(&h->lock);
 }
 
-/* src/bspline_create.c:1864-1877 dispatches on sp_code; this library only
- * creates 3-D multi-splines, which carry a private tail.  Anything else is
+/* src/bspline_create.c:1864-1877 dispatches on sp_code; this library
+ * creates only multi-splines, which carry a private tail.  Anything else is
  * treated as a plain {codes, coefs} object. */
 void destroy_Bspline(void *spline) {
   if (!spline) return;
@@ -576,8 +750,10 @@ void destroy_Bspline(void *spline) {
   if (h->mirror) einspline_mirror_release(h->mirror);
   h->mirror = NULL;
   free(h->pub.s.coefs);
-  if (h->pub.s.tcode == SINGLE_COMPLEX) free(h->pub.c.lapl3);
-  if (h->pub.s.tcode == DOUBLE_COMPLEX) free(h->pub.z.lapl3);
+  if (h->dim == 3 && h->pub.s.tcode == SINGLE_COMPLEX) free(h->pub.c.lapl3);
+  if (h->dim == 3 && h->pub.s.tcode == DOUBLE_COMPLEX) free(h->pub.z.lapl3);
+  if (h->dim == 2 && h->pub.s.tcode == SINGLE_COMPLEX) free(h->pub.c2.lapl2);
+  if (h->dim == 2 && h->pub.s.tcode == DOUBLE_COMPLEX) free(h->pub.z2.lapl2);
   h->magic = 0;
   pthread_mutex_destroy(&h->lock);
   free(h);

@@ -2,17 +2,26 @@
 //
 // With random points, each coefficient row is read by every point whose 4x4x4
 // stencil covers it, at random times during the launch.  When the table is
-// larger than the caches, most of those reads go to HBM.  Sorting the points
+// larger than the caches, most of those reads go to HBM.  Grouping the points
 // by the Morton code of their cell makes points that share rows run close
 // together in time.  The eval kernel then reads `perm[slot]` to find its point
 // and maps blocks so that each XCD (with its own L2) walks one contiguous run
-// of the sorted order.
+// of the grouped order.
 //
-// Results do not depend on the order: each point's outputs are written to its
-// own index by the same instruction sequence, so the traversal changes only
-// which cache serves the reads.
+// The grouping is a counting sort on a coarsened Morton key (about two points
+// per bin, at most 2^11 bins), in two launches:
+//   bin_points    key of every point and its rank in its bin (atomic count);
+//   scan_scatter  every block computes the exclusive prefix sum of the counts
+//                 in LDS (8 KB of L2 hits) and places its own points:
+//                 perm[offset[bin] + rank] = point.
+// No block waits for another and there is no device-wide fence (on a
+// multi-XCD GPU a device-scope release writes back the L2).  The counts are
+// double-buffered: while one call uses one buffer, its block 0 clears the
+// other for the next call.
+// The order inside a bin depends on atomic timing.  Results do not: each
+// point's outputs are written to its own index by the same instruction
+// sequence, so the traversal changes only which cache serves the reads.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <mutex>
@@ -20,6 +29,10 @@
 #include "runtime_internal.hpp"
 
 namespace {
+
+constexpr int kMaxBinBits = 11;            // 2048 bins: 8 KB of counts in LDS
+constexpr int kBins = 1 << kMaxBinBits;
+constexpr int kThreads = 256;
 
 // Spread the low 10 bits of v so that bit k lands at bit 3k.
 __device__ __forceinline__ unsigned spread3(unsigned v) {
@@ -38,18 +51,86 @@ __device__ __forceinline__ unsigned cell_of(double x, double x0, double xi, int 
   return (unsigned)c >> shift;
 }
 
+// Inclusive scan of one value per lane across a wavefront.
+__device__ __forceinline__ int wave_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(kThreads) zero_ints(int *__restrict__ p, int n) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 template <class PosT>
-__global__ void __launch_bounds__(256) cell_keys(const PosT *__restrict__ pos, int n, esk::Params P,
-                                                 int shift, unsigned *__restrict__ keys,
-                                                 int *__restrict__ idx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kThreads) bin_points(const PosT *__restrict__ pos, int n, esk::Params P,
+                                                       int cell_shift, int bin_shift,
+                                                       int *__restrict__ counts, int *__restrict__ bin,
+                                                       int *__restrict__ rank) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const PosT *p = pos + 3l * i;
-  const unsigned cx = cell_of((double)p[0], P.x0, P.xi, P.nx4, shift);
-  const unsigned cy = cell_of((double)p[1], P.y0, P.yi, P.ny4, shift);
-  const unsigned cz = cell_of((double)p[2], P.z0, P.zi, P.nz4, shift);
-  keys[i] = (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
-  idx[i] = i;
+  const unsigned cx = cell_of((double)p[0], P.x0, P.xi, P.nx4, cell_shift);
+  const unsigned cy = cell_of((double)p[1], P.y0, P.yi, P.ny4, cell_shift);
+  const unsigned cz = cell_of((double)p[2], P.z0, P.zi, P.nz4, cell_shift);
+  const int b = (int)(((spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz)) >> bin_shift);
+  bin[i] = b;
+  rank[i] = atomicAdd(&counts[b], 1);
+}
+
+// Exclusive prefix sum of counts[0, nbins) into offs[] (LDS), by every block:
+// rounds of kThreads consecutive bins (one per thread, no LDS bank
+// conflicts), each scanned across the wavefront with shuffles and across the
+// block's four wavefronts through LDS, with a running carry between rounds.
+__device__ __forceinline__ void block_offsets(const int *__restrict__ counts, int nbins, int *offs,
+                                              int *wave_sum) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kBins / kThreads; ++k) {
+    const int b = k * kThreads + t;
+    if (b < nbins) offs[b] = counts[b];
+  }
+  __syncthreads();
+  int carry = 0;
+  for (int k = 0; k < nbins; k += kThreads) {
+    const int b = k + t;
+    const int v = b < nbins ? offs[b] : 0;
+    const int incl = wave_scan(v);
+    if ((t & 63) == 63) wave_sum[t >> 6] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      const int ws = wave_sum[w];
+      if (w < (t >> 6)) before += ws;
+      total += ws;
+    }
+    if (b < nbins) offs[b] = carry + before + incl - v;
+    carry += total;
+    __syncthreads();                                      // wave_sum is rewritten next round
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) scan_scatter(const int *__restrict__ counts,
+                                                         int *__restrict__ next_counts, int nbins,
+                                                         const int *__restrict__ bin,
+                                                         const int *__restrict__ rank, int n,
+                                                         int *__restrict__ perm) {
+  __shared__ int offs[kBins];
+  __shared__ int wave_sum[kThreads / 64];
+  if (blockIdx.x == 0) {
+    // the next call counts into the other buffer; clear it now
+#pragma unroll
+    for (int k = 0; k < kBins / kThreads; ++k) next_counts[k * kThreads + threadIdx.x] = 0;
+  }
+  block_offsets(counts, nbins, offs, wave_sum);
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) perm[offs[bin[i]] + rank[i]] = i;
 }
 
 int bits_for(int n4) {
@@ -63,9 +144,9 @@ int bits_for(int n4) {
 struct EskOrderWs {
   std::mutex mu;
   int device = -1;
-  char *buf = nullptr;          // keys[2][cap], idx[2][cap], sort scratch
+  int *buf = nullptr;           // counts[2][kBins], bin[cap], rank[cap], perm[cap]
   size_t cap = 0;               // points the buffers hold
-  size_t temp_bytes = 0;        // sort scratch bytes
+  int parity = 0;               // which counts buffer the next call uses (the other is zero)
   hipEvent_t done = nullptr;    // recorded after the last launch that read the buffers
 };
 
@@ -88,63 +169,63 @@ int esk_order_begin(EskOrderWs **slot, const esk::Params &P, const void *pos, bo
     w = *slot;
   }
   w->mu.lock();                 // released by esk_order_end
-  const int bx = bits_for(P.nx4), by = bits_for(P.ny4), bz = bits_for(P.nz4);
-  const int bmax = std::max(bx, std::max(by, bz));
-  const int shift = bmax > 10 ? bmax - 10 : 0;   // 30-bit keys: coarser cells on huge grids
-  const int end_bit = 3 * (bmax - shift);
-  size_t temp = 0;
-  e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned *)nullptr, (unsigned *)nullptr,
-                                         (const int *)nullptr, (int *)nullptr, num, 0, end_bit, s);
-  if (e != hipSuccess) { w->mu.unlock(); return esk_hip_fail(e, "sort size query"); }
-  temp = (temp + 255) & ~(size_t)255;
-  const size_t need_pts = ((size_t)num + 63) & ~(size_t)63;
+  auto bail = [&](int rc) { w->mu.unlock(); return rc; };
   int dev = 0;
   (void)hipGetDevice(&dev);
-  if (w->done && w->device != dev) {
-    w->mu.unlock();
-    return esk_fail(EINSPLINE_EINVAL, "locality order: table used from another device");
-  }
+  if (w->done && w->device != dev)
+    return bail(esk_fail(EINSPLINE_EINVAL, "locality order: table used from another device"));
   if (!w->done) {
-    if ((e = hipEventCreateWithFlags(&w->done, hipEventDisableTiming)) != hipSuccess) {
-      w->mu.unlock();
-      return esk_hip_fail(e, "hipEventCreate");
-    }
+    if ((e = hipEventCreateWithFlags(&w->done, hipEventDisableTiming)) != hipSuccess)
+      return bail(esk_hip_fail(e, "hipEventCreate"));
     w->device = dev;
     // recorded once so that every later wait and synchronise has a completed event to see
-    if ((e = hipEventRecord(w->done, s)) != hipSuccess) {
-      w->mu.unlock();
-      return esk_hip_fail(e, "hipEventRecord");
-    }
+    if ((e = hipEventRecord(w->done, s)) != hipSuccess) return bail(esk_hip_fail(e, "hipEventRecord"));
   }
-  if (need_pts > w->cap || temp > w->temp_bytes) {
+  // order this stream after the previous user of the buffers
+  if ((e = hipStreamWaitEvent(s, w->done, 0)) != hipSuccess) return bail(esk_hip_fail(e, "hipStreamWaitEvent"));
+  constexpr size_t kHead = 2 * (size_t)kBins;            // two count buffers
+  const size_t need = ((size_t)num + 63) & ~(size_t)63;
+  if (need > w->cap) {
     // earlier launches on any stream may still read the old buffers
     (void)hipEventSynchronize(w->done);
     if (w->buf) (void)hipFree(w->buf);
     w->buf = nullptr;
-    const size_t cap = std::max(need_pts, w->cap), tb = std::max(temp, w->temp_bytes);
-    if ((e = hipMalloc(&w->buf, cap * 16 + tb)) != hipSuccess) {
-      w->cap = w->temp_bytes = 0;
-      w->mu.unlock();
-      return esk_fail(EINSPLINE_ENOMEM, "locality order: cannot allocate %zu bytes", cap * 16 + tb);
-    }
-    w->cap = cap;
-    w->temp_bytes = tb;
+    w->cap = 0;
+    const size_t ints = kHead + 3 * need;
+    if ((e = hipMalloc(&w->buf, ints * sizeof(int))) != hipSuccess)
+      return bail(esk_fail(EINSPLINE_ENOMEM, "locality order: cannot allocate %zu bytes", ints * sizeof(int)));
+    w->cap = need;
+    hipLaunchKernelGGL(zero_ints, dim3((unsigned)((kHead + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       w->buf, (int)kHead);
+    if ((e = hipGetLastError()) != hipSuccess) return bail(esk_hip_fail(e, "zero launch"));
   }
-  // order this stream after the previous user of the buffers
-  if ((e = hipStreamWaitEvent(s, w->done, 0)) != hipSuccess) { w->mu.unlock(); return esk_hip_fail(e, "hipStreamWaitEvent"); }
-  unsigned *k0 = (unsigned *)w->buf, *k1 = k0 + w->cap;
-  int *i0 = (int *)(k1 + w->cap), *i1 = i0 + w->cap;
-  void *tmp = (void *)(i1 + w->cap);
-  const int blocks = (num + 255) / 256;
+  int *counts = w->buf + (size_t)w->parity * kBins, *next_counts = w->buf + (size_t)(1 - w->parity) * kBins;
+  int *bin = w->buf + kHead, *rank = bin + w->cap, *pm = rank + w->cap;
+  // key: Morton code of the cell (30 bits at most; coarser cells on huge grids),
+  // then the top bin_bits of it
+  const int bmax = std::max(bits_for(P.nx4), std::max(bits_for(P.ny4), bits_for(P.nz4)));
+  const int cell_shift = bmax > 10 ? bmax - 10 : 0;
+  const int key_bits = 3 * (bmax - cell_shift);
+  int want = 0;
+  while ((1 << (want + 1)) <= num) ++want;                 // about two points per bin
+  const int bin_bits = std::max(0, std::min(std::min(want, key_bits), kMaxBinBits));
+  const int bin_shift = key_bits - bin_bits;
+  const int nbins = 1 << bin_bits;
+  const int blocks = (num + kThreads - 1) / kThreads;
   if (pos_f32)
-    hipLaunchKernelGGL(cell_keys<float>, dim3(blocks), dim3(256), 0, s, (const float *)pos, num, P, shift, k0, i0);
+    hipLaunchKernelGGL(bin_points<float>, dim3(blocks), dim3(kThreads), 0, s, (const float *)pos, num, P,
+                       cell_shift, bin_shift, counts, bin, rank);
   else
-    hipLaunchKernelGGL(cell_keys<double>, dim3(blocks), dim3(256), 0, s, (const double *)pos, num, P, shift, k0, i0);
-  if ((e = hipGetLastError()) != hipSuccess) { w->mu.unlock(); return esk_hip_fail(e, "cell_keys launch"); }
-  size_t tb = w->temp_bytes;
-  e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, i1, num, 0, end_bit, s);
-  if (e != hipSuccess) { w->mu.unlock(); return esk_hip_fail(e, "radix sort"); }
-  *perm = i1;
+    hipLaunchKernelGGL(bin_points<double>, dim3(blocks), dim3(kThreads), 0, s, (const double *)pos, num, P,
+                       cell_shift, bin_shift, counts, bin, rank);
+  hipLaunchKernelGGL(scan_scatter, dim3(blocks), dim3(kThreads), 0, s, counts, next_counts, nbins, bin, rank,
+                     num, pm);
+  if ((e = hipGetLastError()) != hipSuccess) {
+    w->cap = 0;                 // the counts may be dirty: reallocate and clear on the next call
+    return bail(esk_hip_fail(e, "locality order launch"));
+  }
+  w->parity = 1 - w->parity;
+  *perm = pm;
   return EINSPLINE_OK;
 }
 

@@ -241,6 +241,7 @@ static multi_UBspline_3d_hip *upload(const void *src, int src_float, size_t src_
   p->storage = st;
   p->bytes = bytes;
   p->device = dev;
+  p->dim = 3;
   t->lane_bytes = 0;
   t->block = 0;
   t->store_policy = esk::STORE_STAGED_NT;
@@ -270,6 +271,8 @@ static void dims_of(const H *s, int &Nx, int &Ny, int &Nz) {
 #define UPLOAD_FROM(S, IS_FLOAT, NCOMP, TC, ST)                                       \
   do {                                                                                \
     if (!S || !S->coefs) { fail(EINSPLINE_EINVAL, "NULL spline"); return nullptr; }   \
+    const einspline_host3d *hh_ = einspline_host_of((void *)S);                        \
+    if (hh_ && hh_->dim != 3) { fail(EINSPLINE_EINVAL, "spline is not 3-D"); return nullptr; } \
     int Nx, Ny, Nz;                                                                   \
     dims_of(S, Nx, Ny, Nz);                                                           \
     return upload(S->coefs, IS_FLOAT, (size_t)S->z_stride * NCOMP, Nx, Ny, Nz,        \
@@ -300,6 +303,66 @@ extern "C" multi_UBspline_3d_z_hip *create_multi_UBspline_3d_z_hip(multi_UBsplin
   UPLOAD_FROM(s, 0, 2, DOUBLE_COMPLEX, EINSPLINE_STORE_F64);
 }
 
+// 1-D and 2-D host tables: the same device layout with Ny (and Nz) = 1.
+static const Ugrid kNoAxis = {0.0, 1.0, 1, 1.0, 0.0};
+
+template <class H>
+static multi_UBspline_3d_hip *upload12(const H *s, int dim, int is_float, int ncomp, type_code tc,
+                                       einspline_storage st, const Ugrid &yg, intptr_t row_elems,
+                                       int Ny) {
+  if (!s || !s->coefs) { fail(EINSPLINE_EINVAL, "NULL spline"); return nullptr; }
+  const einspline_host3d *h = einspline_host_of((void *)s);
+  if (h && h->dim != dim) { fail(EINSPLINE_EINVAL, "spline has %d dimensions, not %d", h->dim, dim); return nullptr; }
+  const int Nx = h ? h->Nx
+                   : s->x_grid.num + ((s->xBC.lCode == PERIODIC || s->xBC.lCode == ANTIPERIODIC) ? 3 : 2);
+  multi_UBspline_3d_hip *t = upload(s->coefs, is_float, (size_t)row_elems * ncomp, Nx, Ny, 1,
+                                    s->num_splines, ncomp, s->x_grid, yg, kNoAxis, tc, st);
+  if (t) t->dim = dim;
+  return t;
+}
+#define UPLOAD_1D(S, IS_FLOAT, NCOMP, TC, ST) \
+  return upload12(S, 1, IS_FLOAT, NCOMP, TC, ST, kNoAxis, S ? S->x_stride : 0, 1)
+#define UPLOAD_2D(S, IS_FLOAT, NCOMP, TC, ST)                                             \
+  return upload12(S, 2, IS_FLOAT, NCOMP, TC, ST, S ? S->y_grid : kNoAxis, S ? S->y_stride : 0, \
+                  S ? (int)(S->x_stride / S->y_stride) : 0)
+
+extern "C" multi_UBspline_1d_s_hip *create_multi_UBspline_1d_s_hip(multi_UBspline_1d_s *s) {
+  UPLOAD_1D(s, 1, 1, SINGLE_REAL, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_1d_s_hip *create_multi_UBspline_1d_s_hip_conv(multi_UBspline_1d_d *s) {
+  UPLOAD_1D(s, 0, 1, SINGLE_REAL, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_1d_d_hip *create_multi_UBspline_1d_d_hip(multi_UBspline_1d_d *s) {
+  UPLOAD_1D(s, 0, 1, DOUBLE_REAL, EINSPLINE_STORE_F64);
+}
+extern "C" multi_UBspline_1d_c_hip *create_multi_UBspline_1d_c_hip(multi_UBspline_1d_c *s) {
+  UPLOAD_1D(s, 1, 2, SINGLE_COMPLEX, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_1d_c_hip *create_multi_UBspline_1d_c_hip_conv(multi_UBspline_1d_z *s) {
+  UPLOAD_1D(s, 0, 2, SINGLE_COMPLEX, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_1d_z_hip *create_multi_UBspline_1d_z_hip(multi_UBspline_1d_z *s) {
+  UPLOAD_1D(s, 0, 2, DOUBLE_COMPLEX, EINSPLINE_STORE_F64);
+}
+extern "C" multi_UBspline_2d_s_hip *create_multi_UBspline_2d_s_hip(multi_UBspline_2d_s *s) {
+  UPLOAD_2D(s, 1, 1, SINGLE_REAL, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_2d_s_hip *create_multi_UBspline_2d_s_hip_conv(multi_UBspline_2d_d *s) {
+  UPLOAD_2D(s, 0, 1, SINGLE_REAL, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_2d_d_hip *create_multi_UBspline_2d_d_hip(multi_UBspline_2d_d *s) {
+  UPLOAD_2D(s, 0, 1, DOUBLE_REAL, EINSPLINE_STORE_F64);
+}
+extern "C" multi_UBspline_2d_c_hip *create_multi_UBspline_2d_c_hip(multi_UBspline_2d_c *s) {
+  UPLOAD_2D(s, 1, 2, SINGLE_COMPLEX, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_2d_c_hip *create_multi_UBspline_2d_c_hip_conv(multi_UBspline_2d_z *s) {
+  UPLOAD_2D(s, 0, 2, SINGLE_COMPLEX, EINSPLINE_STORE_F32);
+}
+extern "C" multi_UBspline_2d_z_hip *create_multi_UBspline_2d_z_hip(multi_UBspline_2d_z *s) {
+  UPLOAD_2D(s, 0, 2, DOUBLE_COMPLEX, EINSPLINE_STORE_F64);
+}
+
 extern "C" void destroy_multi_UBspline_3d_hip(multi_UBspline_3d_hip *t) {
   if (!t) return;
   esk_order_free(((TableExt *)t)->order_ws);
@@ -317,6 +380,9 @@ extern "C" int einspline_set_tile(multi_UBspline_3d_hip *t, int lane_bytes, int 
   if (lane_bytes && lane_bytes != 8 && lane_bytes != 16 && lane_bytes != 32)
     return fail(EINSPLINE_EINVAL, "lane_bytes must be 8, 16 or 32");
   if (wide && lane_bytes == 8) return fail(EINSPLINE_EINVAL, "F64 tables need lane_bytes >= 16");
+  const bool narrow = t->storage == EINSPLINE_STORE_BF16 || t->storage == EINSPLINE_STORE_F16;
+  if (narrow && lane_bytes == 32)
+    return fail(EINSPLINE_EINVAL, "16-bit tables take lane_bytes 8 or 16");
   if (block && (block < 64 || block > 256 || block % 64))
     return fail(EINSPLINE_EINVAL, "block_threads must be 64, 128, 192 or 256");
   TableExt *x = (TableExt *)t;
@@ -398,13 +464,14 @@ void tune_save() {
 }  // namespace
 
 static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
-                       void *v, void *g, void *h, int packed, void *stream);
+                       void *v, void *g, void *h, int packed, void *stream, void *gh = nullptr);
 static int result_reals(const multi_UBspline_3d_hip *t);
 
 extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample, int *lb_out,
                                   int *blk_out, int *nt_out, float *ms_out) {
   // nt_out receives the chosen store policy
   if (!t || mode < 0 || mode > 3) return fail(EINSPLINE_EINVAL, "bad autotune arguments");
+  if (t->dim != 3) return fail(EINSPLINE_EINVAL, "the autotuner takes 3-D tables");
   if (sample <= 0) sample = 8192;
   const int eb = (int)storage_bytes(t->storage);
   const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb, sample};
@@ -453,9 +520,10 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   const int old_order = ext->point_order;
   int best_lb = 16, best_blk = 256, best_nt = esk::STORE_STAGED_NT, best_order = EINSPLINE_ORDER_GIVEN;
   float best = -1.0f, default_ms = -1.0f;
-  const int lbs_f64[] = {16, 32}, lbs_other[] = {8, 16, 32};
-  const int *lbs = t->storage == EINSPLINE_STORE_F64 ? lbs_f64 : lbs_other;
-  const int nlb = t->storage == EINSPLINE_STORE_F64 ? 2 : 3;
+  const int lbs_f64[] = {16, 32}, lbs_f32[] = {8, 16, 32}, lbs_16[] = {8, 16};
+  const bool narrow = t->storage == EINSPLINE_STORE_BF16 || t->storage == EINSPLINE_STORE_F16;
+  const int *lbs = t->storage == EINSPLINE_STORE_F64 ? lbs_f64 : narrow ? lbs_16 : lbs_f32;
+  const int nlb = narrow || t->storage == EINSPLINE_STORE_F64 ? 2 : 3;
   const int blks[] = {64, 128, 256};
   const int norders = sample >= kOrderMinPoints ? 2 : 1;
   for (int order = 0; order < norders; ++order)
@@ -591,15 +659,67 @@ static int order_begin(const multi_UBspline_3d_hip *t, Params &P, const void *po
   return EINSPLINE_OK;
 }
 
-static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
-                       void *v, void *g, void *h, int packed, void *stream) {
+// 1-D and 2-D tables (FP32 / FP64 storage): strided or pointer-array
+// outputs, 16-byte lanes, points in the given order.  pos holds dim
+// coordinates per point.  row_stride (reals) places the Laplacian row after
+// the gradient row when the pointer form has no Laplacian array.
+static int run12(const multi_UBspline_3d_hip *t, int mode, int api, const void *pos, int num,
+                 void *v, void *g, void *h, long row_stride, hipStream_t stream) {
+  if (t->storage != EINSPLINE_STORE_F32 && t->storage != EINSPLINE_STORE_F64)
+    return fail(EINSPLINE_EINVAL, "1-D and 2-D tables are stored as F32 or F64");
+  if (mode < esk::MODE_V || mode > esk::MODE_VGH)
+    return fail(EINSPLINE_EINVAL, "mode not available for %d-D tables", t->dim);
   Launch L;
   Params P;
   int rc = esk_prepare(t, num, L, P);
   if (rc) return rc;
   if (num == 0) return EINSPLINE_OK;
-  if (!pos || !v || (mode >= esk::MODE_VG && !g) || (mode >= esk::MODE_VGL && !h))
+  if (!pos || !v || (mode >= esk::MODE_VG && !g) ||
+      (mode >= esk::MODE_VGL && !h && !(api != esk::API_STRIDED && row_stride > 0)))
     return fail(EINSPLINE_EINVAL, "NULL position or output pointer");
+  const int eb = (int)storage_bytes(t->storage);
+  L.lane_bytes = 16;
+  P.chunks = t->row_reals * eb / 16;
+  if ((long long)num * P.chunks >= (1ll << 31))
+    return fail(EINSPLINE_EINVAL, "batch too large: points * lanes per point must be < 2^31");
+  L.mode = mode;
+  L.api = api;
+  P.pos = pos;
+  P.out_v = v; P.out_g = g; P.out_h = h;
+  P.ld_v = P.nreal;
+  P.ld_g = (long)t->dim * P.nreal;
+  P.ld_h = (mode == esk::MODE_VGL || t->dim == 1) ? (long)P.nreal : 4l * P.nreal;
+  P.row_stride = row_stride;
+  hipError_t e = esk::launch12(L, P, t->dim, t->storage == EINSPLINE_STORE_F64, stream);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return EINSPLINE_OK;
+}
+
+static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
+                       void *v, void *g, void *h, int packed, void *stream, void *gh) {
+  if (t && t->dim != 3) {
+    if (packed) return fail(EINSPLINE_EINVAL, "packed Hessians are for 3-D tables");
+    return run12(t, mode, esk::API_STRIDED, pos, num, v, g, h, 0, (hipStream_t)stream);
+  }
+  Launch L;
+  Params P;
+  int rc = esk_prepare(t, num, L, P);
+  if (rc) return rc;
+  if (num == 0) return EINSPLINE_OK;
+  if (!pos || !v || (mode >= esk::MODE_VG && !g) || (mode >= esk::MODE_VGL && !h) ||
+      (mode == esk::MODE_VGHGH && !gh))
+    return fail(EINSPLINE_EINVAL, "NULL position or output pointer");
+  if (mode == esk::MODE_VGHGH) {
+    // the vghgh kernels exist for lanes of at most 16 bytes and 4 reals
+    const int eb = (int)storage_bytes(t->storage);
+    while (L.lane_bytes > 16 || L.lane_bytes / eb > 4) {
+      P.chunks *= 2;
+      L.lane_bytes /= 2;
+    }
+    packed = 0;
+    P.out_gh = gh;
+    P.ld_gh = 27l * P.nreal;
+  }
   L.mode = mode;
   L.api = esk::API_STRIDED;
   P.pos = pos;
@@ -607,6 +727,8 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
   P.ld_v = P.nreal;
   P.ld_g = 3l * P.nreal;
   P.ld_h = mode == esk::MODE_VGL ? (long)P.nreal : (packed ? 6l : 9l) * P.nreal;
+  if ((long long)num * P.chunks >= (1ll << 31))
+    return fail(EINSPLINE_EINVAL, "batch too large: points * lanes per point must be < 2^31");
   P.hpacked = packed ? 1 : 0;
   EskOrderWs *ws;
   if ((rc = order_begin(t, P, pos, false, num, (hipStream_t)stream, &ws))) return rc;
@@ -632,6 +754,10 @@ extern "C" int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *t, const do
                                         void *v, void *g, void *h, int packed, void *stream) {
   return run_strided(t, esk::MODE_VGH, pos, num, v, g, h, packed, stream);
 }
+extern "C" int einspline_eval_vghgh_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
+                                          void *v, void *g, void *h, void *gh, void *stream) {
+  return run_strided(t, esk::MODE_VGHGH, pos, num, v, g, h, 0, stream, gh);
+}
 
 // CUDA-style: per-point pointer arrays, packed Hessian, synchronous.
 static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, const void *pos,
@@ -641,6 +767,7 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
     return fail(EINSPLINE_EINVAL, "device table has the wrong type for this entry point");
   if (t && want == SINGLE_REAL && t->storage == EINSPLINE_STORE_F64)
     return fail(EINSPLINE_EINVAL, "F64 storage in a single-precision table");
+  if (t && t->dim != 3) return fail(EINSPLINE_EINVAL, "this entry point takes a 3-D table");
   Launch L;
   Params P;
   int rc = esk_prepare(t, num, L, P);
@@ -663,10 +790,10 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
   P.linv = linv;
   P.sign = sign;
   P.row_stride = (long)row_stride * (L.cplx ? 2 : 1);
-  EskOrderWs *ws;
-  if ((rc = order_begin(t, P, pos, pos_f32, num, nullptr, &ws))) return rc;
+  // The pointer-array entry points evaluate in the given order whatever the
+  // table's point order: their locality-order form faulted the GPU once in
+  // testing with no cause found, so it stays switched off.
   hipError_t e = dispatch(t, L, P, nullptr);
-  if (ws) esk_order_end(ws, nullptr);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
@@ -762,6 +889,68 @@ extern "C" int eval_multi_multi_UBspline_3d_z_vgl_hip(multi_UBspline_3d_z_hip *t
                   nullptr, num, Linv, nullptr, row_stride);
 }
 
+// 1-D CUDA-style entry points (src/multi_bspline_eval_cuda.h:10-51): one
+// coordinate per point, per-point output pointers, synchronous.
+static int run_ptrs12(const multi_UBspline_3d_hip *t, type_code want, int mode, const void *pos,
+                      bool pos_f32, void **v, void **g, void **h, int num, int row_stride) {
+  if (t && t->tcode != want)
+    return fail(EINSPLINE_EINVAL, "device table has the wrong type for this entry point");
+  if (t && t->dim != 1) return fail(EINSPLINE_EINVAL, "this entry point takes a 1-D table");
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL device table");
+  if (mode == esk::MODE_VGL && !h && row_stride < t->num_splines)
+    return fail(EINSPLINE_EINVAL, "row_stride %d is smaller than num_splines %d", row_stride,
+                t->num_splines);
+  const int ncomp = (want == SINGLE_COMPLEX || want == DOUBLE_COMPLEX) ? 2 : 1;
+  int rc = run12(t, mode, pos_f32 ? esk::API_PTRS_F32POS : esk::API_PTRS_F64POS, pos, num, v, g, h,
+                 (long)row_stride * ncomp, nullptr);
+  if (rc || num == 0) return rc;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+  return EINSPLINE_OK;
+}
+
+extern "C" int eval_multi_multi_UBspline_1d_s_hip(multi_UBspline_1d_s_hip *t, float *pos, float *vals[],
+                                                  int num) {
+  return run_ptrs12(t, SINGLE_REAL, esk::MODE_V, pos, true, (void **)vals, nullptr, nullptr, num, 0);
+}
+extern "C" int eval_multi_multi_UBspline_1d_s_vgl_hip(multi_UBspline_1d_s_hip *t, float *pos,
+                                                      float *vals[], float *grads[], float *lapl[],
+                                                      int num) {
+  if (!lapl) return fail(EINSPLINE_EINVAL, "NULL Laplacian pointer array");
+  return run_ptrs12(t, SINGLE_REAL, esk::MODE_VGL, pos, true, (void **)vals, (void **)grads,
+                    (void **)lapl, num, 0);
+}
+extern "C" int eval_multi_multi_UBspline_1d_d_hip(multi_UBspline_1d_d_hip *t, double *pos,
+                                                  double *vals[], int num) {
+  return run_ptrs12(t, DOUBLE_REAL, esk::MODE_V, pos, false, (void **)vals, nullptr, nullptr, num, 0);
+}
+extern "C" int eval_multi_multi_UBspline_1d_d_vgl_hip(multi_UBspline_1d_d_hip *t, double *pos,
+                                                      double *vals[], double *grad_lapl[], int num,
+                                                      int row_stride) {
+  return run_ptrs12(t, DOUBLE_REAL, esk::MODE_VGL, pos, false, (void **)vals, (void **)grad_lapl,
+                    nullptr, num, row_stride);
+}
+extern "C" int eval_multi_multi_UBspline_1d_c_hip(multi_UBspline_1d_c_hip *t, float *pos,
+                                                  complex_float *vals[], int num) {
+  return run_ptrs12(t, SINGLE_COMPLEX, esk::MODE_V, pos, true, (void **)vals, nullptr, nullptr, num, 0);
+}
+extern "C" int eval_multi_multi_UBspline_1d_c_vgl_hip(multi_UBspline_1d_c_hip *t, float *pos,
+                                                      complex_float *vals[], complex_float *grad_lapl[],
+                                                      int num, int row_stride) {
+  return run_ptrs12(t, SINGLE_COMPLEX, esk::MODE_VGL, pos, true, (void **)vals, (void **)grad_lapl,
+                    nullptr, num, row_stride);
+}
+extern "C" int eval_multi_multi_UBspline_1d_z_hip(multi_UBspline_1d_z_hip *t, double *pos,
+                                                  complex_double *vals[], int num) {
+  return run_ptrs12(t, DOUBLE_COMPLEX, esk::MODE_V, pos, false, (void **)vals, nullptr, nullptr, num, 0);
+}
+extern "C" int eval_multi_multi_UBspline_1d_z_vgl_hip(multi_UBspline_1d_z_hip *t, double *pos,
+                                                      complex_double *vals[], complex_double *grad_lapl[],
+                                                      int num, int row_stride) {
+  return run_ptrs12(t, DOUBLE_COMPLEX, esk::MODE_VGL, pos, false, (void **)vals, (void **)grad_lapl,
+                    nullptr, num, row_stride);
+}
+
 // ---------------------------------------------------------------------------
 // single-point path: einspline's original API, one point per call
 
@@ -824,11 +1013,20 @@ const multi_UBspline_3d_hip *mirror_of(void *spline, const char *fn) {
     if (h->mirror) destroy_multi_UBspline_3d_hip((multi_UBspline_3d_hip *)h->mirror);
     h->mirror = nullptr;
     multi_UBspline_3d_hip *m = nullptr;
-    switch (h->pub.s.tcode) {
-      case SINGLE_REAL: m = create_multi_UBspline_3d_s_hip(&h->pub.s); break;
-      case DOUBLE_REAL: m = create_multi_UBspline_3d_d_hip(&h->pub.d); break;
-      case SINGLE_COMPLEX: m = create_multi_UBspline_3d_c_hip(&h->pub.c); break;
-      case DOUBLE_COMPLEX: m = create_multi_UBspline_3d_z_hip(&h->pub.z); break;
+    const int key = h->dim * 4 + (int)h->pub.s.tcode;
+    switch (key) {
+      case 3 * 4 + SINGLE_REAL: m = create_multi_UBspline_3d_s_hip(&h->pub.s); break;
+      case 3 * 4 + DOUBLE_REAL: m = create_multi_UBspline_3d_d_hip(&h->pub.d); break;
+      case 3 * 4 + SINGLE_COMPLEX: m = create_multi_UBspline_3d_c_hip(&h->pub.c); break;
+      case 3 * 4 + DOUBLE_COMPLEX: m = create_multi_UBspline_3d_z_hip(&h->pub.z); break;
+      case 2 * 4 + SINGLE_REAL: m = create_multi_UBspline_2d_s_hip(&h->pub.s2); break;
+      case 2 * 4 + DOUBLE_REAL: m = create_multi_UBspline_2d_d_hip(&h->pub.d2); break;
+      case 2 * 4 + SINGLE_COMPLEX: m = create_multi_UBspline_2d_c_hip(&h->pub.c2); break;
+      case 2 * 4 + DOUBLE_COMPLEX: m = create_multi_UBspline_2d_z_hip(&h->pub.z2); break;
+      case 1 * 4 + SINGLE_REAL: m = create_multi_UBspline_1d_s_hip(&h->pub.s1); break;
+      case 1 * 4 + DOUBLE_REAL: m = create_multi_UBspline_1d_d_hip(&h->pub.d1); break;
+      case 1 * 4 + SINGLE_COMPLEX: m = create_multi_UBspline_1d_c_hip(&h->pub.c1); break;
+      case 1 * 4 + DOUBLE_COMPLEX: m = create_multi_UBspline_1d_z_hip(&h->pub.z1); break;
     }
     if (!m) { pthread_mutex_unlock(&h->lock); die(fn); }
     h->mirror = m;
@@ -842,16 +1040,20 @@ const multi_UBspline_3d_hip *mirror_of(void *spline, const char *fn) {
 // Evaluate one point and copy `ncomp` result blocks of `elt`-byte result
 // elements back: values (N), gradients (3N), Laplacian (N) or Hessian (9N).
 void eval_point(void *spline, const char *fn, int mode, double x, double y, double z,
-                void *vals, void *grads, void *lh) {
+                void *vals, void *grads, void *lh, void *gh = nullptr) {
   const multi_UBspline_3d_hip *t = mirror_of(spline, fn);
   const size_t elt = (t->storage == EINSPLINE_STORE_F64 ? 8 : 4) *
                      ((t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX) ? 2 : 1);
-  const size_t N = (size_t)t->num_splines;
-  const size_t nv = N * elt, ng = mode >= esk::MODE_VG ? 3 * N * elt : 0;
-  const size_t nh = mode == esk::MODE_VGH ? 9 * N * elt : mode == esk::MODE_VGL ? N * elt : 0;
-  // layout in the staging buffer: [pos | vals | grads | lapl/hess], 256-byte aligned parts
+  const size_t N = (size_t)t->num_splines, dim = (size_t)t->dim;
+  const size_t nv = N * elt, ng = mode >= esk::MODE_VG ? dim * N * elt : 0;
+  const size_t hh = dim == 3 ? 9 : dim == 2 ? 4 : 1;    // 1-D vgh is the vgl
+  const size_t nh = mode == esk::MODE_VGH || mode == esk::MODE_VGHGH ? hh * N * elt
+                  : mode == esk::MODE_VGL ? N * elt : 0;
+  const size_t n3 = mode == esk::MODE_VGHGH ? 27 * N * elt : 0;
+  // layout in the staging buffer: [pos | vals | grads | lapl/hess | third], 256-byte aligned parts
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t o_v = 256, o_g = o_v + up(nv), o_h = o_g + up(ng), total = o_h + up(nh);
+  const size_t o_v = 256, o_g = o_v + up(nv), o_h = o_g + up(ng), o_3 = o_h + up(nh),
+               total = o_3 + up(n3);
   if (stage_reserve(total)) die(fn);
   unsigned char *hb = (unsigned char *)g_stage.host, *db = (unsigned char *)g_stage.dev;
   double *hp = (double *)hb;
@@ -859,7 +1061,7 @@ void eval_point(void *spline, const char *fn, int mode, double x, double y, doub
   hipError_t e = hipMemcpyAsync(db, hb, 3 * sizeof(double), hipMemcpyHostToDevice, g_stage.stream);
   if (e != hipSuccess) { hip_fail(e, "hipMemcpyAsync"); die(fn); }
   int rc = run_strided(t, mode, (const double *)db, 1, db + o_v, ng ? db + o_g : nullptr,
-                       nh ? db + o_h : nullptr, 0, g_stage.stream);
+                       nh ? db + o_h : nullptr, 0, g_stage.stream, n3 ? db + o_3 : nullptr);
   if (rc) die(fn);
   e = hipMemcpyAsync(hb + o_v, db + o_v, total - o_v, hipMemcpyDeviceToHost, g_stage.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stage.stream);
@@ -867,6 +1069,7 @@ void eval_point(void *spline, const char *fn, int mode, double x, double y, doub
   memcpy(vals, hb + o_v, nv);
   if (ng) memcpy(grads, hb + o_g, ng);
   if (nh) memcpy(lh, hb + o_h, nh);
+  if (n3) memcpy(gh, hb + o_3, n3);
 }
 
 }  // namespace
@@ -887,12 +1090,56 @@ void eval_point(void *spline, const char *fn, int mode, double x, double y, doub
   extern "C" void eval_multi_UBspline_3d_##T##_vgh(multi_UBspline_3d_##T *s, double x,        \
                                                    double y, double z, TY *v, TY *g, TY *h) {  \
     eval_point(s, "eval_multi_UBspline_3d_" #T "_vgh", esk::MODE_VGH, x, y, z, v, g, h);      \
+  }                                                                                            \
+  extern "C" void eval_multi_UBspline_3d_##T##_vghgh(multi_UBspline_3d_##T *s, double x,      \
+                                                     double y, double z, TY *v, TY *g, TY *h, \
+                                                     TY *gh) {                                 \
+    eval_point(s, "eval_multi_UBspline_3d_" #T "_vghgh", esk::MODE_VGHGH, x, y, z, v, g, h, gh); \
   }
 
 POINT_API(s, float)
 POINT_API(d, double)
 POINT_API(c, complex_float)
 POINT_API(z, complex_double)
+
+// 1-D and 2-D (src/multi_bspline_eval_std_d_impl.h:37-476 and the s/c/z files)
+#define POINT_API12(T, TY)                                                                      \
+  extern "C" void eval_multi_UBspline_1d_##T(multi_UBspline_1d_##T *s, double x, TY *v) {       \
+    eval_point(s, "eval_multi_UBspline_1d_" #T, esk::MODE_V, x, 0, 0, v, nullptr, nullptr);     \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_1d_##T##_vg(multi_UBspline_1d_##T *s, double x, TY *v,   \
+                                                  TY *g) {                                      \
+    eval_point(s, "eval_multi_UBspline_1d_" #T "_vg", esk::MODE_VG, x, 0, 0, v, g, nullptr);    \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_1d_##T##_vgl(multi_UBspline_1d_##T *s, double x, TY *v,  \
+                                                   TY *g, TY *l) {                              \
+    eval_point(s, "eval_multi_UBspline_1d_" #T "_vgl", esk::MODE_VGL, x, 0, 0, v, g, l);        \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_1d_##T##_vgh(multi_UBspline_1d_##T *s, double x, TY *v,  \
+                                                   TY *g, TY *h) {                              \
+    eval_point(s, "eval_multi_UBspline_1d_" #T "_vgh", esk::MODE_VGH, x, 0, 0, v, g, h);        \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_2d_##T(multi_UBspline_2d_##T *s, double x, double y,     \
+                                             TY *v) {                                           \
+    eval_point(s, "eval_multi_UBspline_2d_" #T, esk::MODE_V, x, y, 0, v, nullptr, nullptr);     \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_2d_##T##_vg(multi_UBspline_2d_##T *s, double x, double y, \
+                                                  TY *v, TY *g) {                               \
+    eval_point(s, "eval_multi_UBspline_2d_" #T "_vg", esk::MODE_VG, x, y, 0, v, g, nullptr);    \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_2d_##T##_vgl(multi_UBspline_2d_##T *s, double x,         \
+                                                   double y, TY *v, TY *g, TY *l) {             \
+    eval_point(s, "eval_multi_UBspline_2d_" #T "_vgl", esk::MODE_VGL, x, y, 0, v, g, l);        \
+  }                                                                                             \
+  extern "C" void eval_multi_UBspline_2d_##T##_vgh(multi_UBspline_2d_##T *s, double x,         \
+                                                   double y, TY *v, TY *g, TY *h) {             \
+    eval_point(s, "eval_multi_UBspline_2d_" #T "_vgh", esk::MODE_VGH, x, y, 0, v, g, h);        \
+  }
+
+POINT_API12(s, float)
+POINT_API12(d, double)
+POINT_API12(c, complex_float)
+POINT_API12(z, complex_double)
 
 // ---------------------------------------------------------------------------
 // plain copy kernel: the bandwidth ceiling and the counter calibration for
@@ -915,6 +1162,27 @@ extern "C" int einspline_hip_stream_copy(void *dst, const void *src, size_t byte
                      (const u32x4 *)src, (u32x4 *)dst, n);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "stream copy launch");
+}
+
+__global__ void __launch_bounds__(256) stream_fill_kernel(u32x4 *__restrict__ dst, unsigned v, size_t n,
+                                                          int nt) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const u32x4 q = {v, v, v, v};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (nt) __builtin_nontemporal_store(q, dst + i);
+    else dst[i] = q;
+  }
+}
+
+extern "C" int einspline_hip_stream_fill(void *dst, unsigned value, size_t bytes, int nontemporal,
+                                         void *stream) {
+  if (!dst || bytes % 16) return fail(EINSPLINE_EINVAL, "fill needs 16-byte multiples");
+  const size_t n = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(stream_fill_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     (u32x4 *)dst, value, n, nontemporal ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "stream fill launch");
 }
 
 // ---------------------------------------------------------------------------

@@ -89,6 +89,17 @@ serve_fn group_kernel(int g) {
   }
 }
 
+// Dynamic LDS of a group kernel: room for staged stores in every wavefront
+// (requests keep their table's store policy).
+size_t group_lds(int g) {
+  switch (g) {
+    case 0: return esk::StageLds<float, 16, esk::MODE_VGH, true>::bytes(256, esk::STORE_STAGED);
+    case 2: return esk::StageLds<esk::bf16s, 16, esk::MODE_VGH, true>::bytes(256, esk::STORE_STAGED);
+    case 4: return esk::StageLds<esk::f16s, 16, esk::MODE_VGH, true>::bytes(256, esk::STORE_STAGED);
+    default: return 0;   // several points per wavefront: no staging
+  }
+}
+
 int storage_group(einspline_storage st) {
   return st == EINSPLINE_STORE_F32 ? 0 : st == EINSPLINE_STORE_BF16 ? 2 : 4;
 }
@@ -204,8 +215,14 @@ int einspline_server::init(const einspline_server_config *c) {
   e = hipGetDeviceProperties(&prop, device);
   if (e != hipSuccess) return esk_hip_fail(e, "hipGetDeviceProperties");
   for (int g = 0; g < kGroups; ++g) {
+    const size_t lds = group_lds(g);
+    if (lds > 65536) {
+      e = hipFuncSetAttribute((const void *)group_kernel(g), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+      if (e != hipSuccess) return esk_hip_fail(e, "hipFuncSetAttribute");
+    }
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)group_kernel(g), 256, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)group_kernel(g), 256, lds);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     blocks[g] = prop.multiProcessorCount * per_cu;
   }
@@ -233,7 +250,7 @@ hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
   const BatchDesc *d = (const BatchDesc *)s.dbuf;
   for (int g = 0; g < kGroups; ++g)
     if (mask & (1u << g))
-      hipLaunchKernelGGL(group_kernel(g), dim3(blocks[g]), dim3(256), 0, s.stream, d, g);
+      hipLaunchKernelGGL(group_kernel(g), dim3(blocks[g]), dim3(256), group_lds(g), s.stream, d, g);
   if (hipStreamEndCapture(s.stream, &graph) != hipSuccess) return nullptr;
   hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -532,6 +549,7 @@ extern "C" int einspline_server_register_table(einspline_server *s, void *host,
   if (!s || !host || !table_id) return esk_fail(EINSPLINE_EINVAL, "NULL argument");
   einspline_host3d *h = einspline_host_of(host);
   if (!h) return esk_fail(EINSPLINE_EINVAL, "spline was not created by this library");
+  if (h->dim != 3) return esk_fail(EINSPLINE_EINVAL, "serving takes 3-D tables");
   Table t;
   t.host = host;
   t.storage = storage;

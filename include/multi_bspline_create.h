@@ -1,8 +1,9 @@
 /*
- * multi_bspline_create.h -- construction of 3-D uniform multi-splines.
+ * multi_bspline_create.h -- construction of 1-D, 2-D and 3-D uniform
+ * multi-splines.
  *
- * Replaces: src/multi_bspline_create.h:51-67 (3d_s), :84-103 (3d_d),
- * :119-135 (3d_c), :151-171 (3d_z); destroy via src/bspline_base.h:98-102.
+ * Replaces: src/multi_bspline_create.h:36-174 (1d/2d/3d for s, d, c, z);
+ * destroy via src/bspline_base.h:98-102.
  *
  * create_* allocates the descriptor and an (uninitialised) coefficient
  * table; set_* solves the interpolation problem for one orbital and writes
@@ -72,6 +73,47 @@ void set_multi_UBspline_3d_c_block(multi_UBspline_3d_c *spline, int first,
                                    int count, const complex_float *data);
 void set_multi_UBspline_3d_z_block(multi_UBspline_3d_z *spline, int first,
                                    int count, const complex_double *data);
+
+/* ---- 1-D and 2-D ------------------------------------------------------
+ * Same rules per axis as 3-D.  data is T[Mx] (1-D) or T[Mx][My] (2-D). */
+
+/* src/multi_bspline_create.c:88-139, :141-147 */
+multi_UBspline_1d_s *create_multi_UBspline_1d_s(Ugrid x_grid, BCtype_s xBC, int num_splines);
+void set_multi_UBspline_1d_s(multi_UBspline_1d_s *spline, int spline_num, float *data);
+/* src/multi_bspline_create.c:152-214, :216-250 */
+multi_UBspline_2d_s *create_multi_UBspline_2d_s(Ugrid x_grid, Ugrid y_grid, BCtype_s xBC,
+                                                BCtype_s yBC, int num_splines);
+void set_multi_UBspline_2d_s(multi_UBspline_2d_s *spline, int spline_num, float *data);
+
+/* src/multi_bspline_create.c:767-820, :822-828 */
+multi_UBspline_1d_d *create_multi_UBspline_1d_d(Ugrid x_grid, BCtype_d xBC, int num_splines);
+void set_multi_UBspline_1d_d(multi_UBspline_1d_d *spline, int spline_num, double *data);
+/* src/multi_bspline_create.c:830-837: set with this orbital's own boundary conditions */
+void set_multi_UBspline_1d_d_BC(multi_UBspline_1d_d *spline, int spline_num, double *data,
+                                BCtype_d xBC);
+/* src/multi_bspline_create.c:840-901, :903-935 */
+multi_UBspline_2d_d *create_multi_UBspline_2d_d(Ugrid x_grid, Ugrid y_grid, BCtype_d xBC,
+                                                BCtype_d yBC, int num_splines);
+void set_multi_UBspline_2d_d(multi_UBspline_2d_d *spline, int spline_num, double *data);
+
+/* src/multi_bspline_create.c:389-432, :434-454 */
+multi_UBspline_1d_c *create_multi_UBspline_1d_c(Ugrid x_grid, BCtype_c xBC, int num_splines);
+void set_multi_UBspline_1d_c(multi_UBspline_1d_c *spline, int spline_num, complex_float *data);
+/* src/multi_bspline_create.c:456-520, :522-574 */
+multi_UBspline_2d_c *create_multi_UBspline_2d_c(Ugrid x_grid, Ugrid y_grid, BCtype_c xBC,
+                                                BCtype_c yBC, int num_splines);
+void set_multi_UBspline_2d_c(multi_UBspline_2d_c *spline, int spline_num, complex_float *data);
+
+/* src/multi_bspline_create.c:1091-1136, :1138-1166 */
+multi_UBspline_1d_z *create_multi_UBspline_1d_z(Ugrid x_grid, BCtype_z xBC, int num_splines);
+void set_multi_UBspline_1d_z(multi_UBspline_1d_z *spline, int spline_num, complex_double *data);
+/* src/multi_bspline_create.c:1168-1197 */
+void set_multi_UBspline_1d_z_BC(multi_UBspline_1d_z *spline, int spline_num,
+                                complex_double *data, BCtype_z xBC);
+/* src/multi_bspline_create.c:1199-1255, :1257-1315 */
+multi_UBspline_2d_z *create_multi_UBspline_2d_z(Ugrid x_grid, Ugrid y_grid, BCtype_z xBC,
+                                                BCtype_z yBC, int num_splines);
+void set_multi_UBspline_2d_z(multi_UBspline_2d_z *spline, int spline_num, complex_double *data);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,31 @@ void eval_multi_UBspline_3d_c_vgh(multi_UBspline_3d_c *spline,
                                    double x, double y, double z,
                                    complex_float *vals, complex_float *grads, complex_float *hess);
 
+/* Value, gradient, Hessian and the 27 third derivatives (gradhess[27n + 9a +
+ * 3b + c] = d3 f_n / dx_a dx_b dx_c, full symmetric tensor).
+ * Replaces: src/multi_bspline_eval_std_c_impl.h:864-1045; the reference's std_c.h:861 declares it under the s name by mistake. */
+void eval_multi_UBspline_3d_c_vghgh(multi_UBspline_3d_c *spline,
+                                     double x, double y, double z,
+                                     complex_float *vals, complex_float *grads, complex_float *hess, complex_float *gradhess);
+
+/* ---- 1-D and 2-D (replaces src/multi_bspline_eval_c.h:31-88; src/multi_bspline_eval_std_c_impl.h:38-475) ----
+ * 1-D: vals[N], grads[N], lapl[N]; the 1-D "vgh" is the vgl (hess[N] holds
+ * the second derivative), as in the reference.  2-D: vals[N], grads[2N]
+ * ([n][2]), lapl[N], hess[4N] (row-major symmetric 2x2 per orbital). */
+void eval_multi_UBspline_1d_c(multi_UBspline_1d_c *spline, double x, complex_float *vals);
+void eval_multi_UBspline_1d_c_vg(multi_UBspline_1d_c *spline, double x, complex_float *vals, complex_float *grads);
+void eval_multi_UBspline_1d_c_vgl(multi_UBspline_1d_c *spline, double x, complex_float *vals, complex_float *grads,
+                                  complex_float *lapl);
+void eval_multi_UBspline_1d_c_vgh(multi_UBspline_1d_c *spline, double x, complex_float *vals, complex_float *grads,
+                                  complex_float *hess);
+void eval_multi_UBspline_2d_c(multi_UBspline_2d_c *spline, double x, double y, complex_float *vals);
+void eval_multi_UBspline_2d_c_vg(multi_UBspline_2d_c *spline, double x, double y, complex_float *vals,
+                                 complex_float *grads);
+void eval_multi_UBspline_2d_c_vgl(multi_UBspline_2d_c *spline, double x, double y, complex_float *vals,
+                                  complex_float *grads, complex_float *lapl);
+void eval_multi_UBspline_2d_c_vgh(multi_UBspline_2d_c *spline, double x, double y, complex_float *vals,
+                                  complex_float *grads, complex_float *hess);
+
 #ifdef __cplusplus
 }
 #endif

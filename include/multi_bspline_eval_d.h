@@ -45,6 +45,31 @@ void eval_multi_UBspline_3d_d_vgh(multi_UBspline_3d_d *spline,
                                    double x, double y, double z,
                                    double *vals, double *grads, double *hess);
 
+/* Value, gradient, Hessian and the 27 third derivatives (gradhess[27n + 9a +
+ * 3b + c] = d3 f_n / dx_a dx_b dx_c, full symmetric tensor).
+ * Replaces: src/multi_bspline_eval_std_d_impl.h:868-1049 (sse: _sse_d_impl.h:1381). */
+void eval_multi_UBspline_3d_d_vghgh(multi_UBspline_3d_d *spline,
+                                     double x, double y, double z,
+                                     double *vals, double *grads, double *hess, double *gradhess);
+
+/* ---- 1-D and 2-D (replaces src/multi_bspline_eval_d.h:31-86; src/multi_bspline_eval_std_d_impl.h:38-478) ----
+ * 1-D: vals[N], grads[N], lapl[N]; the 1-D "vgh" is the vgl (hess[N] holds
+ * the second derivative), as in the reference.  2-D: vals[N], grads[2N]
+ * ([n][2]), lapl[N], hess[4N] (row-major symmetric 2x2 per orbital). */
+void eval_multi_UBspline_1d_d(multi_UBspline_1d_d *spline, double x, double *vals);
+void eval_multi_UBspline_1d_d_vg(multi_UBspline_1d_d *spline, double x, double *vals, double *grads);
+void eval_multi_UBspline_1d_d_vgl(multi_UBspline_1d_d *spline, double x, double *vals, double *grads,
+                                  double *lapl);
+void eval_multi_UBspline_1d_d_vgh(multi_UBspline_1d_d *spline, double x, double *vals, double *grads,
+                                  double *hess);
+void eval_multi_UBspline_2d_d(multi_UBspline_2d_d *spline, double x, double y, double *vals);
+void eval_multi_UBspline_2d_d_vg(multi_UBspline_2d_d *spline, double x, double y, double *vals,
+                                 double *grads);
+void eval_multi_UBspline_2d_d_vgl(multi_UBspline_2d_d *spline, double x, double y, double *vals,
+                                  double *grads, double *lapl);
+void eval_multi_UBspline_2d_d_vgh(multi_UBspline_2d_d *spline, double x, double y, double *vals,
+                                  double *grads, double *hess);
+
 #ifdef __cplusplus
 }
 #endif

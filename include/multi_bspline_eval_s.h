@@ -45,6 +45,31 @@ void eval_multi_UBspline_3d_s_vgh(multi_UBspline_3d_s *spline,
                                    double x, double y, double z,
                                    float *vals, float *grads, float *hess);
 
+/* Value, gradient, Hessian and the 27 third derivatives (gradhess[27n + 9a +
+ * 3b + c] = d3 f_n / dx_a dx_b dx_c, full symmetric tensor).
+ * Replaces: src/multi_bspline_eval_std_s_impl.h:866-1047 (sse: _sse_s_impl.h:1285). */
+void eval_multi_UBspline_3d_s_vghgh(multi_UBspline_3d_s *spline,
+                                     double x, double y, double z,
+                                     float *vals, float *grads, float *hess, float *gradhess);
+
+/* ---- 1-D and 2-D (replaces src/multi_bspline_eval_s.h:29-84; src/multi_bspline_eval_std_s_impl.h:38-478) ----
+ * 1-D: vals[N], grads[N], lapl[N]; the 1-D "vgh" is the vgl (hess[N] holds
+ * the second derivative), as in the reference.  2-D: vals[N], grads[2N]
+ * ([n][2]), lapl[N], hess[4N] (row-major symmetric 2x2 per orbital). */
+void eval_multi_UBspline_1d_s(multi_UBspline_1d_s *spline, double x, float *vals);
+void eval_multi_UBspline_1d_s_vg(multi_UBspline_1d_s *spline, double x, float *vals, float *grads);
+void eval_multi_UBspline_1d_s_vgl(multi_UBspline_1d_s *spline, double x, float *vals, float *grads,
+                                  float *lapl);
+void eval_multi_UBspline_1d_s_vgh(multi_UBspline_1d_s *spline, double x, float *vals, float *grads,
+                                  float *hess);
+void eval_multi_UBspline_2d_s(multi_UBspline_2d_s *spline, double x, double y, float *vals);
+void eval_multi_UBspline_2d_s_vg(multi_UBspline_2d_s *spline, double x, double y, float *vals,
+                                 float *grads);
+void eval_multi_UBspline_2d_s_vgl(multi_UBspline_2d_s *spline, double x, double y, float *vals,
+                                  float *grads, float *lapl);
+void eval_multi_UBspline_2d_s_vgh(multi_UBspline_2d_s *spline, double x, double y, float *vals,
+                                  float *grads, float *hess);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,8 @@ typedef struct multi_UBspline_3d_hip {
   einspline_storage storage;
   size_t    bytes;                       /* device bytes of coefs */
   int       device;                      /* HIP device ordinal */
+  int       dim;                         /* 3, or 1 / 2 for tables from the 1d / 2d creates
+                                            (Ny, Nz = 1 past dim) */
 } multi_UBspline_3d_hip;
 
 typedef multi_UBspline_3d_hip multi_UBspline_3d_s_hip;
@@ -92,6 +94,63 @@ multi_UBspline_3d_s_hip *create_multi_UBspline_3d_s_hip_store(const multi_UBspli
 
 /* Free a device table (the reference has none; its callers cudaFree by hand). */
 void destroy_multi_UBspline_3d_hip(multi_UBspline_3d_hip *table);
+
+/* ---- 1-D and 2-D device tables -----------------------------------------
+ * src/multi_bspline_create_cuda.h:10-52 (create_multi_UBspline_{1,2}d_X_cuda
+ * and the _conv forms).  The device table has the 3-D descriptor with dim 1
+ * or 2; FP32 (s, c) or FP64 (d, z) storage.  They evaluate through the
+ * strided einspline_eval_{v,vg,vgl,vgh}_batch calls below, with `dim`
+ * coordinates per point in pos_d and per-point outputs vals N, grads dim*N,
+ * lapl N, hess 4N (2-D) or N (1-D, where vgh is vgl).  The tile setting
+ * applies; the point order is always the given one (the locality order,
+ * the autotuner, vghgh and packed Hessians are for 3-D tables). */
+typedef multi_UBspline_3d_hip multi_UBspline_1d_s_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_1d_d_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_1d_c_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_1d_z_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_2d_s_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_2d_d_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_2d_c_hip;
+typedef multi_UBspline_3d_hip multi_UBspline_2d_z_hip;
+multi_UBspline_1d_s_hip *create_multi_UBspline_1d_s_hip(multi_UBspline_1d_s *spline);
+multi_UBspline_1d_s_hip *create_multi_UBspline_1d_s_hip_conv(multi_UBspline_1d_d *spline);
+multi_UBspline_1d_d_hip *create_multi_UBspline_1d_d_hip(multi_UBspline_1d_d *spline);
+multi_UBspline_1d_c_hip *create_multi_UBspline_1d_c_hip(multi_UBspline_1d_c *spline);
+multi_UBspline_1d_c_hip *create_multi_UBspline_1d_c_hip_conv(multi_UBspline_1d_z *spline);
+multi_UBspline_1d_z_hip *create_multi_UBspline_1d_z_hip(multi_UBspline_1d_z *spline);
+multi_UBspline_2d_s_hip *create_multi_UBspline_2d_s_hip(multi_UBspline_2d_s *spline);
+multi_UBspline_2d_s_hip *create_multi_UBspline_2d_s_hip_conv(multi_UBspline_2d_d *spline);
+multi_UBspline_2d_d_hip *create_multi_UBspline_2d_d_hip(multi_UBspline_2d_d *spline);
+multi_UBspline_2d_c_hip *create_multi_UBspline_2d_c_hip(multi_UBspline_2d_c *spline);
+multi_UBspline_2d_c_hip *create_multi_UBspline_2d_c_hip_conv(multi_UBspline_2d_z *spline);
+multi_UBspline_2d_z_hip *create_multi_UBspline_2d_z_hip(multi_UBspline_2d_z *spline);
+
+/* 1-D CUDA-style batched evaluation (src/multi_bspline_eval_cuda.h:10-51):
+ * pos_d[i] is point i (float for s/c, double for d/z); vals_d[i],
+ * grads_d[i], lapl_d[i] / grad_lapl_d[i] are device pointers to point i's
+ * outputs.  The d/c/z vgl form writes the gradient row (N elements) and the
+ * Laplacian row_stride elements after it.  Derivatives are in physical units
+ * for every orbital (the reference's s kernel leaves them in grid units and
+ * skips the last N % 64 orbitals' derivatives).  Synchronous. */
+int eval_multi_multi_UBspline_1d_s_hip(multi_UBspline_1d_s_hip *spline, float *pos_d, float *vals_d[],
+                                       int num);
+int eval_multi_multi_UBspline_1d_s_vgl_hip(multi_UBspline_1d_s_hip *spline, float *pos_d,
+                                           float *vals_d[], float *grads_d[], float *lapl_d[], int num);
+int eval_multi_multi_UBspline_1d_d_hip(multi_UBspline_1d_d_hip *spline, double *pos_d, double *vals_d[],
+                                       int num);
+int eval_multi_multi_UBspline_1d_d_vgl_hip(multi_UBspline_1d_d_hip *spline, double *pos_d,
+                                           double *vals_d[], double *grad_lapl_d[], int num,
+                                           int row_stride);
+int eval_multi_multi_UBspline_1d_c_hip(multi_UBspline_1d_c_hip *spline, float *pos_d,
+                                       complex_float *vals_d[], int num);
+int eval_multi_multi_UBspline_1d_c_vgl_hip(multi_UBspline_1d_c_hip *spline, float *pos_d,
+                                           complex_float *vals_d[], complex_float *grad_lapl_d[],
+                                           int num, int row_stride);
+int eval_multi_multi_UBspline_1d_z_hip(multi_UBspline_1d_z_hip *spline, double *pos_d,
+                                       complex_double *vals_d[], int num);
+int eval_multi_multi_UBspline_1d_z_vgl_hip(multi_UBspline_1d_z_hip *spline, double *pos_d,
+                                           complex_double *vals_d[], complex_double *grad_lapl_d[],
+                                           int num, int row_stride);
 
 /* ---- CUDA-style batched evaluation -------------------------------------
  * src/multi_bspline_eval_cuda.h:62-137.  pos_d[3*i+{0,1,2}] is point i;
@@ -166,11 +225,18 @@ int einspline_eval_vgl_batch(const multi_UBspline_3d_hip *spline, const double *
 int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
                              void *vals_d, void *grads_d, void *hess_d, int hess_packed,
                              void *stream);
+/* vghgh: as vgh with the full 3x3 Hessian, plus gradhess_d + i*27N holding the
+ * 27 third derivatives of each orbital ([n][a][b][c], symmetric).  Batched
+ * form of eval_multi_UBspline_3d_X_vghgh (src/multi_bspline_eval_std_d_impl.h:868-1049). */
+int einspline_eval_vghgh_batch(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
+                               void *vals_d, void *grads_d, void *hess_d, void *gradhess_d,
+                               void *stream);
 
 /* Tile shape of the batched kernels: bytes of an orbital row one lane loads
- * per stencil row (4, 8, 16 or 32) and work-items per block (64..256, a
- * multiple of 64).  0 selects the built-in default.  Applies to subsequent
- * launches on `spline`; returns EINSPLINE_EINVAL for unsupported shapes. */
+ * per stencil row (8, 16 or 32; F64 tables 16 or 32, 16-bit tables 8 or 16)
+ * and work-items per block (64..256, a multiple of 64).  0 selects the
+ * built-in default.  Applies to subsequent launches on `spline`; returns
+ * EINSPLINE_EINVAL for unsupported shapes. */
 int einspline_set_tile(multi_UBspline_3d_hip *spline, int lane_bytes, int block_threads);
 
 /* Result store policy (results are identical under all three):
@@ -194,7 +260,9 @@ int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
  * owned by the table (allocated on first use, grown with hipMalloc when a
  * larger batch arrives); calls on the same table from different streams are
  * ordered one after another through it.  Under stream capture the given order
- * is used. */
+ * is used.
+ * Applies to the strided einspline_eval_*_batch calls; the pointer-array
+ * entry points (eval_multi_multi_*_hip) always use the given order. */
 enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
 int einspline_set_point_order(multi_UBspline_3d_hip *spline, int order);
 
@@ -241,6 +309,10 @@ float       einspline_hip_event_elapsed_ms(void *start, void *stop);
 /* Plain device-to-device copy kernel (bytes a multiple of 16), used to
  * measure the achievable HBM bandwidth and to calibrate traffic counters. */
 int         einspline_hip_stream_copy(void *dst_d, const void *src_d, size_t bytes, void *stream);
+/* Plain fill kernel (bytes a multiple of 16; streaming stores if nontemporal),
+ * used to measure the achievable HBM write bandwidth. */
+int         einspline_hip_stream_fill(void *dst_d, unsigned value, size_t bytes, int nontemporal,
+                                      void *stream);
 
 #ifdef __cplusplus
 }

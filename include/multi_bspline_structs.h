@@ -1,9 +1,15 @@
 /*
- * multi_bspline_structs.h -- 3-D uniform multi-spline descriptors,
- * binary-compatible with the reference library.
+ * multi_bspline_structs.h -- 1-D, 2-D and 3-D uniform multi-spline
+ * descriptors, binary-compatible with the reference library.
  *
- * Replaces: src/multi_bspline_structs.h:51-60 (3d_s), :88-97 (3d_d),
- * :128-139 (3d_c), :169-180 (3d_z).
+ * Replaces: src/multi_bspline_structs.h:29-38 (1d_s), :40-49 (2d_s),
+ * :51-60 (3d_s), :66-75 (1d_d), :77-86 (2d_d), :88-97 (3d_d),
+ * :104-113 (1d_c), :115-126 (2d_c), :128-139 (3d_c), :145-154 (1d_z),
+ * :156-167 (2d_z), :169-180 (3d_z).
+ *
+ * 1-D and 2-D tables use the same scheme with fewer axes: coefs[ix][n]
+ * (x_stride = num_splines) and coefs[ix][iy][n] (y_stride = num_splines,
+ * x_stride = Ny*y_stride).
  *
  * Coefficients are stored as coefs[ix][iy][iz][n] with the orbital index n
  * fastest: element (ix,iy,iz,n) lives at
@@ -63,5 +69,89 @@ typedef struct {
   int             num_splines;
   complex_double *lapl3;   /* see multi_UBspline_3d_c::lapl3 */
 } multi_UBspline_3d_z;
+
+/* ---- 1-D and 2-D ----------------------------------------------------- */
+
+typedef struct {
+  spline_code spcode;
+  type_code   tcode;
+  float      *coefs;
+  intptr_t    x_stride;
+  Ugrid       x_grid;
+  BCtype_s    xBC;
+  int         num_splines;
+} multi_UBspline_1d_s;
+
+typedef struct {
+  spline_code spcode;
+  type_code   tcode;
+  float      *coefs;
+  intptr_t    x_stride, y_stride;
+  Ugrid       x_grid, y_grid;
+  BCtype_s    xBC, yBC;
+  int         num_splines;
+} multi_UBspline_2d_s;
+
+typedef struct {
+  spline_code spcode;
+  type_code   tcode;
+  double     *coefs;
+  intptr_t    x_stride;
+  Ugrid       x_grid;
+  BCtype_d    xBC;
+  int         num_splines;
+} multi_UBspline_1d_d;
+
+typedef struct {
+  spline_code spcode;
+  type_code   tcode;
+  double     *coefs;
+  intptr_t    x_stride, y_stride;
+  Ugrid       x_grid, y_grid;
+  BCtype_d    xBC, yBC;
+  int         num_splines;
+} multi_UBspline_2d_d;
+
+typedef struct {
+  spline_code    spcode;
+  type_code      tcode;
+  complex_float *coefs;
+  intptr_t       x_stride;
+  Ugrid          x_grid;
+  BCtype_c       xBC;
+  int            num_splines;
+} multi_UBspline_1d_c;
+
+typedef struct {
+  spline_code    spcode;
+  type_code      tcode;
+  complex_float *coefs;
+  intptr_t       x_stride, y_stride;
+  Ugrid          x_grid, y_grid;
+  BCtype_c       xBC, yBC;
+  int            num_splines;
+  complex_float *lapl2;    /* allocated for layout compatibility, never written */
+} multi_UBspline_2d_c;
+
+typedef struct {
+  spline_code     spcode;
+  type_code       tcode;
+  complex_double *coefs;
+  intptr_t        x_stride;
+  Ugrid           x_grid;
+  BCtype_z        xBC;
+  int             num_splines;
+} multi_UBspline_1d_z;
+
+typedef struct {
+  spline_code     spcode;
+  type_code       tcode;
+  complex_double *coefs;
+  intptr_t        x_stride, y_stride;
+  Ugrid           x_grid, y_grid;
+  BCtype_z        xBC, yBC;
+  int             num_splines;
+  complex_double *lapl2;   /* allocated for layout compatibility, never written */
+} multi_UBspline_2d_z;
 
 #endif /* EINSPLINE_MULTI_BSPLINE_STRUCTS_H */

@@ -1,7 +1,9 @@
 """The C-ABI boundary, checked without a GPU: the library loads, exports
 every function include/*.h declares, and the public structs have the
 reference's x86-64 layout (src/bspline_base.h:49-96,
-src/multi_bspline_structs.h:51-180)."""
+src/multi_bspline_structs.h:29-180).  The expected offsets are worked out
+from the reference's definitions (4-byte enums and ints, 8-byte pointers,
+intptr_t and doubles, natural alignment)."""
 import os
 import re
 import subprocess
@@ -59,6 +61,15 @@ int main(void) {
   S(multi_UBspline_3d_d) P(multi_UBspline_3d_d, zBC) P(multi_UBspline_3d_d, num_splines)
   S(multi_UBspline_3d_c) P(multi_UBspline_3d_c, num_splines) P(multi_UBspline_3d_c, lapl3)
   S(multi_UBspline_3d_z) P(multi_UBspline_3d_z, zBC) P(multi_UBspline_3d_z, lapl3)
+  S(multi_UBspline_1d_s) P(multi_UBspline_1d_s, xBC) P(multi_UBspline_1d_s, num_splines)
+  S(multi_UBspline_1d_d) P(multi_UBspline_1d_d, num_splines)
+  S(multi_UBspline_1d_c) P(multi_UBspline_1d_c, num_splines)
+  S(multi_UBspline_1d_z) P(multi_UBspline_1d_z, num_splines)
+  S(multi_UBspline_2d_s) P(multi_UBspline_2d_s, y_grid) P(multi_UBspline_2d_s, yBC)
+  P(multi_UBspline_2d_s, num_splines)
+  S(multi_UBspline_2d_d) P(multi_UBspline_2d_d, num_splines)
+  S(multi_UBspline_2d_c) P(multi_UBspline_2d_c, num_splines) P(multi_UBspline_2d_c, lapl2)
+  S(multi_UBspline_2d_z) P(multi_UBspline_2d_z, yBC) P(multi_UBspline_2d_z, lapl2)
   return 0;
 }
 """
@@ -76,6 +87,20 @@ EXPECTED = {
     "multi_UBspline_3d_c.lapl3": 240,
     "multi_UBspline_3d_z.sizeof": 296, "multi_UBspline_3d_z.zBC": 240,
     "multi_UBspline_3d_z.lapl3": 288,
+    # 1-D: {codes 0,4; coefs 8; x_stride 16; x_grid 24..64; xBC 64; num_splines}
+    "multi_UBspline_1d_s.sizeof": 88, "multi_UBspline_1d_s.xBC": 64,
+    "multi_UBspline_1d_s.num_splines": 80,
+    "multi_UBspline_1d_d.sizeof": 96, "multi_UBspline_1d_d.num_splines": 88,
+    "multi_UBspline_1d_c.sizeof": 96, "multi_UBspline_1d_c.num_splines": 88,
+    "multi_UBspline_1d_z.sizeof": 112, "multi_UBspline_1d_z.num_splines": 104,
+    # 2-D: {codes; coefs 8; strides 16, 24; grids 32, 72; BCs 112..; num_splines[; lapl2]}
+    "multi_UBspline_2d_s.sizeof": 152, "multi_UBspline_2d_s.y_grid": 72,
+    "multi_UBspline_2d_s.yBC": 128, "multi_UBspline_2d_s.num_splines": 144,
+    "multi_UBspline_2d_d.sizeof": 168, "multi_UBspline_2d_d.num_splines": 160,
+    "multi_UBspline_2d_c.sizeof": 176, "multi_UBspline_2d_c.num_splines": 160,
+    "multi_UBspline_2d_c.lapl2": 168,
+    "multi_UBspline_2d_z.sizeof": 208, "multi_UBspline_2d_z.yBC": 152,
+    "multi_UBspline_2d_z.lapl2": 200,
 }
 
 
@@ -149,3 +174,15 @@ def test_no_silent_cpu_fallback_without_a_gpu():
     assert "hip" in str(e.value).lower()
     assert ea.lib.einspline_eval_vgh_batch(None, None, 1, None, None, None, 0, None) == ea.E_INVAL
     assert ea.lib.einspline_hip_malloc(16) is None
+
+
+def test_fortran_thunks_link():
+    """Every C function the reference's Fortran thunks call
+    (src/fmulti_bspline.c:772-901 and the 1-D/2-D ones) is exported, so
+    fmulti_bspline.c rebuilt against include/ links unchanged."""
+    names = open(os.path.join(ROOT, "tests", "golden", "fortran_thunk_calls.txt")).read().split()
+    assert len(names) > 60
+    lib = os.path.join(ROOT, "einspline_amd", "libeinspline.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines()}
+    assert not sorted(set(names) - exported)

@@ -109,3 +109,74 @@ def test_invalid_arguments_return_null():
         ea.HostSpline("d", (1, 5, 5), (0, 0, 0), (1, 1, 1), 2)
     with pytest.raises(ea.EinsplineError):
         ea.HostSpline("s", (5, 5, 5), (0, 0, 0), (1, 1, 1), 0)
+
+
+# ---- 1-D and 2-D tables (src/multi_bspline_create.c:88-250, :389-574,
+# :767-935, :1091-1315) ----------------------------------------------------
+
+LOW_BCS = {
+    "periodic": [(ea.PERIODIC, ea.PERIODIC)] * 2,
+    "antiperiodic": [(ea.ANTIPERIODIC, ea.ANTIPERIODIC)] * 2,
+    "deriv": [(ea.DERIV1, ea.DERIV2, 0.7, -0.4), (ea.FLAT, ea.NATURAL)],
+}
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("name", list(LOW_BCS))
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+def test_low_dim_create_matches_oracle_and_dense(dim, name, kind, rng):
+    bcs = LOW_BCS[name][:dim]
+    if kind in "cz":
+        bcs = [(b[0], b[1], b[2] + 0.25j, b[3] - 0.5j) if len(b) > 2 else b for b in bcs]
+    M, start, end, N = (9, 7)[:dim], (1.5, -2.0)[:dim], (4.0, 3.5)[:dim], 3
+    h = ea.HostSpline(kind, M, start, end, N, bc=bcs)
+    o = oracle.Table(kind, M, start, end, N, bc=bcs)
+    data = rng.uniform(-1, 1, (N,) + M)
+    if kind in "cz":
+        data = data + 1j * rng.uniform(-1, 1, (N,) + M)
+    for n in range(N):
+        h.set(n, data[n])
+        o.set(n, data[n])
+    c = h.coefs().reshape(o.coefs.shape)
+    tol = 2e-6 if kind in "sc" else 1e-13
+    assert rel(c, o.coefs) < tol
+    if kind in "dz":
+        lr = [(b[2], b[3]) if len(b) > 2 else (0.0, 0.0) for b in bcs]
+        view = c.view(np.complex128) if kind == "z" else c
+        for n in range(N):
+            ref = brute.solve_dense(data[n], [b[:2] for b in bcs], o.delta_inv, lr)
+            assert rel(view[..., n], ref) < 1e-13
+    st = h.struct
+    assert st.num_splines == N and st.spcode == (ea.MULTI_U1D if dim == 1 else ea.MULTI_U2D)
+    if dim == 1:
+        assert st.x_stride == N
+    else:
+        assert st.y_stride == N and st.x_stride == h.shape[1] * N
+
+
+def test_1d_bc_setters_use_their_own_conditions(rng):
+    """set_multi_UBspline_1d_d_BC / _1d_z_BC solve with the given BCs
+    (src/multi_bspline_create.c:830-837, :1168-1197)."""
+    M, N = 11, 2
+    table_bc = [(ea.DERIV1, ea.DERIV1, 0.0, 0.0)]
+    own = (ea.DERIV2, ea.DERIV1, 1.5, -0.5)
+    for kind, bct in (("d", ea.BCtype_d), ("z", ea.BCtype_z)):
+        h = ea.HostSpline(kind, (M,), (0.0,), (2.0,), N, bc=table_bc)
+        data = rng.uniform(-1, 1, (N, M)) + (1j * rng.uniform(-1, 1, (N, M)) if kind == "z" else 0)
+        arr = np.ascontiguousarray(data[1], dtype=np.complex128 if kind == "z" else np.float64)
+        bc = ea._bc(kind, *own)
+        getattr(ea.lib, f"set_multi_UBspline_1d_{kind}_BC")(h.ptr, 1, arr.ctypes.data, bc)
+        o = oracle.Table(kind, (M,), (0.0,), (2.0,), N, bc=[own])
+        o.set(1, data[1])
+        c = h.coefs().reshape(o.coefs.shape)
+        assert rel(c[:, 1], o.coefs[:, 1]) < 1e-13
+        assert not c[:, 0].any()
+
+
+def test_low_dim_struct_layout_and_destroy(rng):
+    for dim in (1, 2):
+        for kind in "sdcz":
+            h = ea.HostSpline(kind, (6, 5)[:dim], (0, 0)[:dim], (1, 1)[:dim], 4)
+            h.set(0, rng.uniform(-1, 1, (6, 5)[:dim]))
+            assert h.coefs()[..., 0].any()
+            h.destroy()
