@@ -23,7 +23,31 @@ if not os.path.exists(LIB_PATH):
         f"{LIB_PATH} is missing: build it with `make -C einspline_amd/csrc` "
         "(or __graft_entry__.build())"
     )
-lib = C.CDLL(LIB_PATH)
+
+
+class _PrototypedLib(C.CDLL):
+    """CDLL whose functions refuse to run until their prototype is declared.
+    Without argtypes ctypes passes a Python int as a C int, masked to 32
+    bits, which silently truncates 64-bit device addresses."""
+
+    def __init__(self, path):
+        super().__init__(path)
+        base = self._FuncPtr
+
+        class _Fn(base):
+            _flags_ = base._flags_
+            _restype_ = base._restype_
+
+            def __call__(self, *args):
+                if self.argtypes is None:
+                    name = getattr(self, "__name__", "library function")
+                    raise TypeError(f"{name}: called before its ctypes prototype was declared")
+                return base.__call__(self, *args)
+
+        self._FuncPtr = _Fn
+
+
+lib = _PrototypedLib(LIB_PATH)
 
 # ---------------------------------------------------------------------------
 # types (include/bspline_base.h, include/multi_bspline_structs.h)
@@ -168,11 +192,10 @@ _proto("eval_multi_multi_UBspline_1d_s_vgl_hip", _i, C.POINTER(multi_UBspline_3d
 for k in "dcz":
     _proto(f"eval_multi_multi_UBspline_1d_{k}_vgl_hip", _i, C.POINTER(multi_UBspline_3d_hip),
            _vp, _vp, _vp, _i, _i)
+for k in "sdcz":
     _proto(f"eval_multi_multi_UBspline_3d_{k}_hip", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _i)
     _proto(f"eval_multi_multi_UBspline_3d_{k}_vgh_hip", _i, C.POINTER(multi_UBspline_3d_hip),
            _vp, _vp, _vp, _vp, _i)
-
-for k in "sdcz":
     _proto(f"eval_multi_multi_UBspline_3d_{k}_vgl_hip", _i, C.POINTER(multi_UBspline_3d_hip),
            _vp, _vp, _vp, _vp, _i, _i)
 _proto("eval_multi_multi_UBspline_3d_s_vgl_sign_hip", _i, C.POINTER(multi_UBspline_3d_hip),
@@ -216,6 +239,12 @@ _proto("einspline_hip_event_synchronize", _i, _vp)
 _proto("einspline_hip_event_elapsed_ms", C.c_float, _vp, _vp)
 _proto("einspline_hip_stream_copy", _i, _vp, _vp, _sz, _vp)
 _proto("einspline_hip_stream_fill", _i, _vp, C.c_uint, _sz, _i, _vp)
+# Slater-determinant kernels (include/determinant_update_hip.h)
+for _k in "sd":
+    _proto(f"einspline_update_inverse_{_k}", _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp)
+    _proto(f"einspline_calc_ratios_{_k}", _i, _vp, _vp, _vp, _i, _i, _i, _i, _vp)
+    _proto(f"einspline_fill_ratios_{_k}", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _vp, _vp,
+           _i, _i, _i, _i, _vp)
 
 
 class EinsplineError(RuntimeError):

@@ -790,10 +790,10 @@ static int run_ptrs(const multi_UBspline_3d_hip *t, type_code want, int mode, co
   P.linv = linv;
   P.sign = sign;
   P.row_stride = (long)row_stride * (L.cplx ? 2 : 1);
-  // The pointer-array entry points evaluate in the given order whatever the
-  // table's point order: their locality-order form faulted the GPU once in
-  // testing with no cause found, so it stays switched off.
+  EskOrderWs *ws;
+  if ((rc = order_begin(t, P, pos, pos_f32, num, nullptr, &ws))) return rc;
   hipError_t e = dispatch(t, L, P, nullptr);
+  if (ws) esk_order_end(ws, nullptr);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");

@@ -261,8 +261,8 @@ int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
  * larger batch arrives); calls on the same table from different streams are
  * ordered one after another through it.  Under stream capture the given order
  * is used.
- * Applies to the strided einspline_eval_*_batch calls; the pointer-array
- * entry points (eval_multi_multi_*_hip) always use the given order. */
+ * Applies to the 3-D batched calls: the strided einspline_eval_*_batch
+ * functions and the pointer-array eval_multi_multi_UBspline_3d_*_hip ones. */
 enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
 int einspline_set_point_order(multi_UBspline_3d_hip *spline, int order);
 

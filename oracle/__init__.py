@@ -7,6 +7,9 @@ The product path (einspline_amd) never touches it.
 * liboracle.so (oracle.c): C restatement of the reference's default-build
   create/set and std evaluation (src/multi_bspline_create.c,
   src/bspline_create.c, src/multi_bspline_eval_std_*_impl.h).
+* liboracle.so also restates the Slater-determinant kernels that consume
+  the orbital rows (src/determinant_update.h, src/cuda_walker.cu):
+  update_inverse() and det_ratio() below.
 * brute.py: numpy closed forms used to pin the restatement (textbook B3
   kernel sums, FFT periodic solve, analytic plane waves).
 """
@@ -55,6 +58,13 @@ lib.oracle_basis_d.argtypes = [_d, _vp, _vp, _vp]
 lib.oracle_basis3_d.restype = None
 lib.oracle_basis3_d.argtypes = [_vp]
 lib.oracle_max_threads.restype = _i
+for t, ct in (("f", C.c_float), ("d", C.c_double)):
+    f = getattr(lib, f"oracle_update_inverse_{t}")
+    f.restype = None
+    f.argtypes = [_vp, _vp, _vp, _i, C.c_long, _i]
+    f = getattr(lib, f"oracle_det_ratio_{t}")
+    f.restype = ct
+    f.argtypes = [_vp, _vp, _i, C.c_long, _i]
 
 
 def basis(t: float):
@@ -164,3 +174,29 @@ def shape_results(mode, N, ncomp, v, g, h, gh=None, dim=3):
     if gh is not None:
         out.append(cx(gh).reshape(P, N, 3, 3, 3))
     return tuple(out)
+
+
+def _det_args(Ainv, *vecs):
+    Ainv = np.asarray(Ainv)
+    assert Ainv.ndim == 2 and Ainv.flags.c_contiguous and Ainv.dtype in (np.float32, np.float64)
+    t = "f" if Ainv.dtype == np.float32 else "d"
+    for v in vecs:
+        assert v is None or (v.dtype == Ainv.dtype and v.flags.c_contiguous)
+    return t, Ainv.shape[1]
+
+
+def update_inverse(A, Ainv, u, k, N=None):
+    """Sherman-Morrison update in place (src/determinant_update.h:8-104): row
+    k of A (modified) becomes u, or, with A None, changes by u.  Matrices are
+    row-major with row stride = their second extent; N defaults to it."""
+    t, rs = _det_args(Ainv, A, u)
+    N = rs if N is None else N
+    getattr(lib, f"oracle_update_inverse_{t}")(A.ctypes.data if A is not None else None,
+                                                Ainv.ctypes.data, u.ctypes.data, N, rs, k)
+
+
+def det_ratio(Ainv, u, col, N=None):
+    """sum_i Ainv[i][col] * u[i] (src/cuda_walker.cu:150-185)."""
+    t, rs = _det_args(Ainv, u)
+    N = rs if N is None else N
+    return getattr(lib, f"oracle_det_ratio_{t}")(Ainv.ctypes.data, u.ctypes.data, N, rs, col)

@@ -499,3 +499,53 @@ int oracle_max_threads(void) {
   return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------
+ * Slater-determinant kernels that consume the orbital rows.
+ *
+ * Row k of A (row-major, stride rs) changes; Ainv is its inverse.
+ *   oracle_update_inverse_*: Sherman-Morrison update, as the pair
+ *     update_inverse_cuda1/2 of src/determinant_update.h:8-104 computes it:
+ *     delta = u - A[k] (A given; A[k] = u is written) or delta = u (A NULL,
+ *     src/determinant_update.cu:10-98); d_j = sum_i Ainv[i][j] delta_i summed
+ *     over i = 0..N-1 (:30-55); colk = column k of the old Ainv;
+ *     prefact = -1 / (1 + d_k); Ainv[i][j] += d_j * (prefact * colk_i)
+ *     (:80-103).  The host form src/determinant_update.cu:164-176 is the
+ *     same update on the transpose.
+ *   oracle_det_ratio_*: sum_i Ainv[i][col] * u_i in row order, the ratio
+ *     det(A') / det(A) that calc_ratios computes (src/cuda_walker.cu:150-185).
+ */
+#define ORACLE_SM(T, NAME)                                                        \
+  void NAME(T *A, T *Ainv, const T *u, int N, long rs, int k) {                  \
+    T *delta = (T *)malloc(sizeof(T) * (size_t)N);                               \
+    T *d = (T *)malloc(sizeof(T) * (size_t)N);                                   \
+    T *colk = (T *)malloc(sizeof(T) * (size_t)N);                                \
+    for (int i = 0; i < N; ++i) {                                                \
+      delta[i] = A ? u[i] - A[k * rs + i] : u[i];                                \
+      colk[i] = Ainv[i * rs + k];                                                \
+    }                                                                            \
+    if (A)                                                                       \
+      for (int i = 0; i < N; ++i) A[k * rs + i] = u[i];                          \
+    for (int j = 0; j < N; ++j) {                                                \
+      T s = 0;                                                                   \
+      for (int i = 0; i < N; ++i) s += Ainv[i * rs + j] * delta[i];              \
+      d[j] = s;                                                                  \
+    }                                                                            \
+    const T prefact = (T)-1 / ((T)1 + d[k]);                                     \
+    for (int i = 0; i < N; ++i) {                                                \
+      const T c = prefact * colk[i];                                             \
+      for (int j = 0; j < N; ++j) Ainv[i * rs + j] += d[j] * c;                  \
+    }                                                                            \
+    free(delta); free(d); free(colk);                                            \
+  }
+ORACLE_SM(float, oracle_update_inverse_f)
+ORACLE_SM(double, oracle_update_inverse_d)
+
+#define ORACLE_RATIO(T, NAME)                                                     \
+  T NAME(const T *Ainv, const T *u, int N, long rs, int col) {                   \
+    T s = 0;                                                                     \
+    for (int i = 0; i < N; ++i) s += Ainv[i * rs + col] * u[i];                  \
+    return s;                                                                    \
+  }
+ORACLE_RATIO(float, oracle_det_ratio_f)
+ORACLE_RATIO(double, oracle_det_ratio_d)

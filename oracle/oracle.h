@@ -44,4 +44,12 @@ int oracle_eval_batch(const otable *t, int is_double, const void *coefs, int mod
 void oracle_basis_d(double t, double a[4], double da[4], double d2a[4]);
 void oracle_basis3_d(double d3a[4]);
 int oracle_max_threads(void);
+
+/* Slater-determinant kernels (row-major matrices with row stride rs):
+ * Sherman-Morrison update of Ainv after row k of A becomes u (A non-NULL; A
+ * is updated) or changes by u (A NULL), and the ratio sum_i Ainv[i][col] u_i. */
+void oracle_update_inverse_f(float *A, float *Ainv, const float *u, int N, long rs, int k);
+void oracle_update_inverse_d(double *A, double *Ainv, const double *u, int N, long rs, int k);
+float oracle_det_ratio_f(const float *Ainv, const float *u, int N, long rs, int col);
+double oracle_det_ratio_d(const double *Ainv, const double *u, int N, long rs, int col);
 #endif

@@ -186,3 +186,17 @@ def test_fortran_thunks_link():
     out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True).stdout
     exported = {l.split()[-1] for l in out.splitlines()}
     assert not sorted(set(names) - exported)
+
+
+def test_every_declared_function_has_a_ctypes_prototype():
+    """ctypes passes a Python int to an unprototyped function as a C int
+    masked to 32 bits, which truncates device addresses: every function of
+    include/*.h must be declared before the tests call it, and a call of an
+    undeclared one raises instead of running."""
+    import einspline_amd as ea
+    import einspline_amd.serve  # noqa: F401  (declares the server functions)
+    missing = sorted(n for n in declared_functions() if getattr(ea.lib, n).argtypes is None)
+    assert not missing, missing
+    fresh = ea.lib._FuncPtr(("einspline_hip_device_count", ea.lib))
+    with pytest.raises(TypeError):
+        fresh()

@@ -100,8 +100,6 @@ def test_workspace_growth_and_two_streams(rng):
             np.testing.assert_array_equal(a.reshape(b.shape), b)
 
 
-@pytest.mark.skip(reason="faulted the GPU once (illegal address) with no cause found; the "
-                         "pointer-array entry points now always use the given order")
 @pytest.mark.parametrize("kind", ["s", "z"])
 def test_pointer_array_api_in_locality_order(kind, rng):
     M, N, P = (9, 9, 9), 32, 2500
