@@ -62,12 +62,19 @@ def plane_wave_block(rng, M, count, nwaves=8, kmax=4):
     return out
 
 
+_TABLES = {}
+
+
 def build_table(kind, M, N, seed, block=32):
+    key = (kind, M, N, seed, block)
+    if key in _TABLES:
+        return _TABLES[key]
     rng = np.random.default_rng(seed)
     h = ea.HostSpline(kind, (M, M, M), (0, 0, 0), (1, 1, 1), N)
     for first in range(0, N, block):
         cnt = min(block, N - first)
         h.set_block(first, plane_wave_block(rng, M, cnt))
+    _TABLES[key] = h
     return h
 
 
@@ -400,6 +407,97 @@ def other_configs(args, res):
     res.setdefault("extra", {})["other_configs"] = rows
 
 
+def _ptr_list(base, count, stride_bytes):
+    return ea.DeviceBuffer.from_array(np.array([base.ptr + m * stride_bytes for m in range(count)], np.uint64))
+
+
+def _time_calls(fn, steps, warmup):
+    """ms per call of fn(stream) from HIP events on one stream."""
+    stream = ea.Stream()
+    for _ in range(warmup):
+        ea.check(fn(stream.ptr), "warmup call")
+    stream.synchronize()
+    e0, e1 = ea.Event(), ea.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        ea.check(fn(stream.ptr), "timed call")
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def determinant(args, res):
+    """The Slater-determinant kernels that consume the orbital rows, at QMC
+    sizes (include/determinant_update_hip.h):
+      * Sherman-Morrison update of 256 walkers' 256 x 256 FP32 inverses, and
+        of 1000 x 128 x 128 (the reference prototype's NUM_MATS x MAT_SIZE,
+        src/determinant_update.cu:349-350); HBM-bound: each inverse is read
+        and written once, 2 N^2 x 4 bytes;
+      * the ratios of 1024 walkers, N = 128 (src/cuda_walker.cu:333-340);
+      * the row fill fused with the ratio on the 48^3 x 256 table, 256
+        walkers, against the unfused pair (value batch + ratios)."""
+    rows = {}
+    rng = np.random.default_rng(7)
+    steps = max(10, args.steps)
+    for name, N, num in (("update_f32_n256_w256", 256, 256), ("update_f32_n128_m1000", 128, 1000)):
+        A = np.repeat((np.eye(N) + 0.5 * rng.uniform(-1, 1, (N, N)) / np.sqrt(N))[None], num, 0)
+        Ainv = np.repeat(np.linalg.inv(A[0])[None], num, 0).astype(np.float32)
+        A = A.astype(np.float32)
+        k = N // 3
+        u = A[:, k].copy()                         # A-form with the row it already has: delta = 0,
+        es = 4                                     # the same work every call, the inverse unchanged
+        bufs = {n: ea.DeviceBuffer.from_array(a) for n, a in (("A", A), ("I", Ainv), ("u", u))}
+        dl, ck = ea.DeviceBuffer(num * N * es), ea.DeviceBuffer(num * N * es)
+        lists = {n: _ptr_list(b, num, (N * N if n in "AI" else N) * es) for n, b in bufs.items()}
+        dll, ckl = _ptr_list(dl, num, N * es), _ptr_list(ck, num, N * es)
+        fn = lambda s: ea.lib.einspline_update_inverse_s(lists["A"].ptr, lists["I"].ptr, lists["u"].ptr,
+                                                         dll.ptr, ckl.ptr, N, N, k, num, s)
+        ms = _time_calls(fn, steps, 3)
+        alg = num * (2 * N * N + 5 * N) * es
+        rows[name] = {"ms_per_call": ms, "updates_per_s": num / (ms * 1e-3),
+                      "algorithmic_bytes": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
+                      "frac_hbm": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "note": "two launches: column-k gather, then the one-pass slab update"}
+    # ratios: 1024 walkers, N = 128
+    N, num = 128, 1024
+    Ainv = rng.uniform(-1, 1, (num, N, N)).astype(np.float32)
+    u = rng.uniform(-1, 1, (num, N)).astype(np.float32)
+    dI, du = ea.DeviceBuffer.from_array(Ainv), ea.DeviceBuffer.from_array(u)
+    Il, ul = _ptr_list(dI, num, N * N * 4), _ptr_list(du, num, N * 4)
+    rat = ea.DeviceBuffer(num * 4)
+    ms = _time_calls(lambda s: ea.lib.einspline_calc_ratios_s(Il.ptr, ul.ptr, rat.ptr, N, N, 15, num, s), steps, 3)
+    rows["ratios_f32_n128_w1024"] = {"ms_per_call": ms, "ratios_per_s": num / (ms * 1e-3),
+                                     "note": "column elec of each inverse is a strided gather (one real per row)"}
+    # fused fill + ratio on the headline table
+    host = build_table("s", 48, 256, seed=2024)
+    tab = ea.DeviceTable(host)
+    NS = N = 256
+    num = 256
+    pts = rng.random((num, 3))
+    pos32 = ea.DeviceBuffer.from_array(pts.astype(np.float32))
+    pos64 = ea.DeviceBuffer.from_array(pts)
+    Ainv = rng.uniform(-1, 1, (num, N, N)).astype(np.float32)
+    dI = ea.DeviceBuffer.from_array(Ainv)
+    Il = _ptr_list(dI, num, N * N * 4)
+    rowb = ea.DeviceBuffer(num * NS * 4)
+    rl = _ptr_list(rowb, num, NS * 4)
+    rat = ea.DeviceBuffer(num * 4)
+    fused = _time_calls(lambda s: ea.lib.einspline_fill_ratios_s(tab.ptr, pos32.ptr, rl.ptr, Il.ptr, rat.ptr,
+                                                                 N, N, 7, num, s), steps, 3)
+
+    def unfused(s):
+        rc = ea.lib.einspline_eval_v_batch(tab.ptr, pos64.ptr, num, rowb.ptr, s)
+        return rc or ea.lib.einspline_calc_ratios_s(Il.ptr, rl.ptr, rat.ptr, N, N, 7, num, s)
+    pair = _time_calls(unfused, steps, 3)
+    rows["fill_ratio_f32_48c_n256_w256"] = {"fused_ms": fused, "unfused_ms": pair,
+                                            "speedup": pair / fused,
+                                            "note": "unfused: einspline_eval_v_batch + einspline_calc_ratios_s"}
+    tab.destroy()
+    for k2, r in rows.items():
+        log(f"[bench] {k2}: {json.dumps(r)}")
+    res.setdefault("extra", {})["determinant"] = rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,6 +509,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-form measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the complex / FP64 configurations")
     ap.add_argument("--no-extra", action="store_true", help="skip the packed-Hessian side measurement")
+    ap.add_argument("--no-det", action="store_true", help="skip the determinant kernels")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
     ap.add_argument("--lane-bytes", type=int, default=0)
@@ -433,6 +532,8 @@ def main():
         precision(args, res)
     if not args.no_configs:
         other_configs(args, res)
+    if not args.no_det:
+        determinant(args, res)
     if not args.no_serving:
         serving(args, res)
     ex = res.get("extra", {})
@@ -447,6 +548,10 @@ def main():
         summ["serving_p99_ms_at_50pct"] = r["load_50pct"]["p99_ms"]
         summ["serving_p99_ms_at_90pct"] = r["load_90pct"]["p99_ms"]
         summ["serving_capacity_evals_per_s"] = r["capacity_evals_per_s"]
+    if "determinant" in ex:
+        d = ex["determinant"]
+        summ["det_update_f32_n256_w256_frac_hbm"] = d["update_f32_n256_w256"]["frac_hbm"]
+        summ["det_fill_ratio_fused_speedup"] = d["fill_ratio_f32_48c_n256_w256"]["speedup"]
     if summ:
         ex["summary"] = summ
     print(json.dumps(res), flush=True)

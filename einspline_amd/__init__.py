@@ -239,6 +239,8 @@ _proto("einspline_hip_event_synchronize", _i, _vp)
 _proto("einspline_hip_event_elapsed_ms", C.c_float, _vp, _vp)
 _proto("einspline_hip_stream_copy", _i, _vp, _vp, _sz, _vp)
 _proto("einspline_hip_stream_fill", _i, _vp, C.c_uint, _sz, _i, _vp)
+_proto("einspline_table_save", _i, C.POINTER(multi_UBspline_3d_hip), C.c_char_p)
+_proto("einspline_table_load", C.POINTER(multi_UBspline_3d_hip), C.c_char_p)
 # Slater-determinant kernels (include/determinant_update_hip.h)
 for _k in "sd":
     _proto(f"einspline_update_inverse_{_k}", _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp)
@@ -495,6 +497,22 @@ class DeviceTable:
             p = lib.create_multi_UBspline_3d_s_hip_store(host.ptr, int(storage))
         if not p:
             raise EinsplineError(E_INVAL, "create device table")
+        self._adopt(p)
+
+    @classmethod
+    def load(cls, path):
+        """A table saved by save() (einspline_table_load): no solve, no conversion."""
+        p = lib.einspline_table_load(os.fsencode(path))
+        if not p:
+            raise EinsplineError(E_INVAL, f"einspline_table_load({path})")
+        self = cls.__new__(cls)
+        self._adopt(p)
+        return self
+
+    def save(self, path):
+        check(lib.einspline_table_save(self.ptr, os.fsencode(path)), f"einspline_table_save({path})")
+
+    def _adopt(self, p):
         self.ptr = p
         t = p.contents
         self.N = t.num_splines

@@ -125,27 +125,30 @@ __device__ __forceinline__ void locate(double x, double x0, double xinv, int n4,
 
 // Uniform cubic B-spline basis (power form, rows of the 4x4 basis matrix in
 // src/bspline_data.c:143-193) and its first and second t-derivatives.
+// Horner steps are explicit FMAs: left to contraction, the compiler may fuse
+// a product in one kernel and not in another (a shared subexpression changes
+// its choice), and every kernel must compute the same weights bit for bit.
 template <class A>
 __device__ __forceinline__ void basis0(A t, A w[4]) {
   const A s6 = (A)(1.0 / 6.0), h = (A)0.5, t3 = (A)(2.0 / 3.0);
-  w[0] = ((-s6 * t + h) * t - h) * t + s6;
-  w[1] = (h * t - (A)1) * t * t + t3;
-  w[2] = ((-h * t + h) * t + h) * t + s6;
-  w[3] = s6 * t * t * t;
+  w[0] = fmaA(fmaA(fmaA(-s6, t, h), t, -h), t, s6);
+  w[1] = fmaA(fmaA(h, t, (A)-1) * t, t, t3);
+  w[2] = fmaA(fmaA(fmaA(-h, t, h), t, h), t, s6);
+  w[3] = ((s6 * t) * t) * t;
 }
 template <class A>
 __device__ __forceinline__ void basis1(A t, A w[4]) {
   const A h = (A)0.5, o5 = (A)1.5;
-  w[0] = (-h * t + (A)1) * t - h;
-  w[1] = (o5 * t - (A)2) * t;
-  w[2] = (-o5 * t + (A)1) * t + h;
-  w[3] = h * t * t;
+  w[0] = fmaA(fmaA(-h, t, (A)1), t, -h);
+  w[1] = fmaA(o5, t, (A)-2) * t;
+  w[2] = fmaA(fmaA(-o5, t, (A)1), t, h);
+  w[3] = (h * t) * t;
 }
 template <class A>
 __device__ __forceinline__ void basis2(A t, A w[4]) {
   w[0] = (A)1 - t;
-  w[1] = (A)3 * t - (A)2;
-  w[2] = (A)1 - (A)3 * t;
+  w[1] = fmaA((A)3, t, (A)-2);
+  w[2] = fmaA((A)-3, t, (A)1);
   w[3] = t;
 }
 

@@ -244,15 +244,32 @@ int einspline_server::init(const einspline_server_config *c) {
 hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
   auto it = s.graphs.find(mask);
   if (it != s.graphs.end()) return it->second;
+  // The graph is built node by node rather than captured: while a stream
+  // captures, the HIP runtime refuses legacy-stream work from other threads
+  // of the process (a caller's hipMemcpy), which a server cannot forbid.
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  if (hipGraphCreate(&graph, 0) != hipSuccess) return nullptr;
   const BatchDesc *d = (const BatchDesc *)s.dbuf;
-  for (int g = 0; g < kGroups; ++g)
-    if (mask & (1u << g))
-      hipLaunchKernelGGL(group_kernel(g), dim3(blocks[g]), dim3(256), group_lds(g), s.stream, d, g);
-  if (hipStreamEndCapture(s.stream, &graph) != hipSuccess) return nullptr;
-  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphNode_t prev = nullptr;
+  hipError_t e = hipSuccess;
+  for (int g = 0; g < kGroups && e == hipSuccess; ++g) {
+    if (!(mask & (1u << g))) continue;
+    int group = g;
+    void *args[] = {(void *)&d, (void *)&group};
+    hipKernelNodeParams p = {};
+    p.func = (void *)group_kernel(g);
+    p.gridDim = dim3(blocks[g]);
+    p.blockDim = dim3(256);
+    p.sharedMemBytes = (unsigned)group_lds(g);
+    p.kernelParams = args;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    // one group after another, as launches on one stream would run
+    e = hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+    prev = node;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return nullptr;
   s.graphs[mask] = exec;

@@ -289,6 +289,17 @@ int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_point
 void einspline_hip_invalidate(void *spline);
 
 /* ---- minimal device runtime (so C and ctypes callers need no HIP) ------ */
+/* ---- tables on disk (no reference counterpart: einspline has no table file) */
+/* Save a device table of any dimension and storage to `path`, as the kernels
+ * read it: a 512-byte header (grids, sizes, storage type, tuned tile, point
+ * order) and the coefficient rows byte for byte, with 64-bit hashes of both.
+ * Written to `path`.tmp, then renamed, so a reader never sees a partial file. */
+int einspline_table_save(const multi_UBspline_3d_hip *spline, const char *path);
+/* Load a saved table onto the current device: no solve and no conversion,
+ * the saved tile and point order restored.  NULL for a missing, damaged or
+ * foreign file (see einspline_hip_last_error). */
+multi_UBspline_3d_hip *einspline_table_load(const char *path);
+
 int         einspline_hip_device_count(void);
 const char *einspline_hip_last_error(void);
 void       *einspline_hip_malloc(size_t bytes);             /* NULL on failure */
