@@ -240,6 +240,7 @@ _proto("einspline_hip_event_elapsed_ms", C.c_float, _vp, _vp)
 _proto("einspline_hip_stream_copy", _i, _vp, _vp, _sz, _vp)
 _proto("einspline_hip_stream_fill", _i, _vp, C.c_uint, _sz, _i, _vp)
 _proto("einspline_table_save", _i, C.POINTER(multi_UBspline_3d_hip), C.c_char_p)
+_proto("einspline_build_table_hip", C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _i, _vp)
 _proto("einspline_table_load", C.POINTER(multi_UBspline_3d_hip), C.c_char_p)
 # Slater-determinant kernels (include/determinant_update_hip.h)
 for _k in "sd":
@@ -498,6 +499,22 @@ class DeviceTable:
         if not p:
             raise EinsplineError(E_INVAL, "create device table")
         self._adopt(p)
+
+    @classmethod
+    def build(cls, host: "HostSpline", data, storage=None, stream=None):
+        """Solve the table on the GPU (einspline_build_table_hip) from samples
+        shaped like set_block's, without a host solve; storage as for
+        einspline_build_table_hip (None: the spline's own type)."""
+        arr = host._data(data, host.N)
+        buf = DeviceBuffer.from_array(arr)
+        p = lib.einspline_build_table_hip(host.ptr, buf.ptr, -1 if storage is None else int(storage),
+                                          stream.ptr if stream else None)
+        buf.free()
+        if not p:
+            raise EinsplineError(E_INVAL, "einspline_build_table_hip")
+        self = cls.__new__(cls)
+        self._adopt(p)
+        return self
 
     @classmethod
     def load(cls, path):

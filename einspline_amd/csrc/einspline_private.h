@@ -46,6 +46,21 @@ typedef struct einspline_host3d {
  * pointer was not created by this library. */
 einspline_host3d *einspline_host_of(void *spline);
 
+/* One axis of a 3-D host spline's interpolation operator, for the GPU
+ * solve (build.hip).  `arrays` holds 4*M doubles: the Thomas factors inv,
+ * g, sub and the periodic bordering vector w (multi_create.c axis_op). */
+typedef struct {
+  int M, n, kind;          /* data points, coefficients, 0 periodic / 1 anti / 2 derivative */
+  double sgn, zinv;        /* periodic wrap sign, 1 / Schur complement */
+  double l0, r2;           /* derivative rows: coefficient of c_0 (left), c_{M+1} (right) */
+  double lv[2], rv[2];     /* boundary values per component */
+  double *arrays;          /* malloc'd; release with free() */
+} einspline_axis_export;
+
+/* Fill ax[0..2] for a 3-D host spline.  Returns 0, -1 (not 3-D or an
+ * unsupported boundary pair) or -2 (out of memory). */
+int einspline_export_axes(void *spline, einspline_axis_export ax[3]);
+
 /* Implemented by the HIP runtime: releases a device mirror. */
 void einspline_mirror_release(void *mirror);
 

@@ -92,9 +92,9 @@ static int axis_init(axis_op *op, const Ugrid *grid, int lcode, int rcode) {
     op->sgn = lcode == PERIODIC ? 1.0 : -1.0;
     op->n = M + 3;
     const int m = M - 1;               /* core: unknowns x_0..x_{M-2} */
-    op->inv = malloc(sizeof(double) * m);
-    op->g = malloc(sizeof(double) * m);
-    op->sub = malloc(sizeof(double) * m);
+    op->inv = calloc((size_t)m, sizeof(double));
+    op->g = calloc((size_t)m, sizeof(double));
+    op->sub = calloc((size_t)m, sizeof(double));
     op->w = calloc((size_t)m, sizeof(double));
     if (!op->inv || !op->g || !op->sub || !op->w) { axis_free(op); return -2; }
     for (int r = 0; r < m; ++r) {
@@ -123,9 +123,9 @@ static int axis_init(axis_op *op, const Ugrid *grid, int lcode, int rcode) {
    * the first data row; row M: right condition with c_{M+1} eliminated by
    * the last data row; rows 2..M-1: data rows. */
   const int m = M;
-  op->inv = malloc(sizeof(double) * m);
-  op->g = malloc(sizeof(double) * m);
-  op->sub = malloc(sizeof(double) * m);
+  op->inv = calloc((size_t)m, sizeof(double));
+  op->g = calloc((size_t)m, sizeof(double));
+  op->sub = calloc((size_t)m, sizeof(double));
   if (!op->inv || !op->g || !op->sub) { axis_free(op); return -2; }
   for (int r = 0; r < m; ++r) {
     double sub = ALPHA, dia = BETA, sup = ALPHA;
@@ -724,6 +724,81 @@ SET_2D_REAL(s, float, 1)
 SET_2D_REAL(d, double, 0)
 SET_2D_CPLX(c, complex_float, 1)
 SET_2D_CPLX(z, complex_double, 0)
+
+/* ------------------------------------------------------------------------ */
+/* operator export for the GPU solve                                         */
+
+int einspline_export_axes(void *spline, einspline_axis_export ax[3]) {
+  einspline_host3d *h = einspline_host_of(spline);
+  if (!h || h->dim != 3) return -1;
+  Ugrid g[3];
+  axis_bc b[3];
+  switch (h->pub.s.tcode) {
+    case SINGLE_REAL: {
+      const multi_UBspline_3d_s *S = &h->pub.s;
+      g[0] = S->x_grid; g[1] = S->y_grid; g[2] = S->z_grid;
+      bcs_real(&b[0], S->xBC.lCode, S->xBC.rCode, S->xBC.lVal, S->xBC.rVal);
+      bcs_real(&b[1], S->yBC.lCode, S->yBC.rCode, S->yBC.lVal, S->yBC.rVal);
+      bcs_real(&b[2], S->zBC.lCode, S->zBC.rCode, S->zBC.lVal, S->zBC.rVal);
+      break;
+    }
+    case DOUBLE_REAL: {
+      const multi_UBspline_3d_d *S = &h->pub.d;
+      g[0] = S->x_grid; g[1] = S->y_grid; g[2] = S->z_grid;
+      bcs_real(&b[0], S->xBC.lCode, S->xBC.rCode, S->xBC.lVal, S->xBC.rVal);
+      bcs_real(&b[1], S->yBC.lCode, S->yBC.rCode, S->yBC.lVal, S->yBC.rVal);
+      bcs_real(&b[2], S->zBC.lCode, S->zBC.rCode, S->zBC.lVal, S->zBC.rVal);
+      break;
+    }
+    case SINGLE_COMPLEX: {
+      const multi_UBspline_3d_c *S = &h->pub.c;
+      g[0] = S->x_grid; g[1] = S->y_grid; g[2] = S->z_grid;
+      bcs_cplx(&b[0], S->xBC.lCode, S->xBC.rCode, S->xBC.lVal_r, S->xBC.lVal_i, S->xBC.rVal_r, S->xBC.rVal_i);
+      bcs_cplx(&b[1], S->yBC.lCode, S->yBC.rCode, S->yBC.lVal_r, S->yBC.lVal_i, S->yBC.rVal_r, S->yBC.rVal_i);
+      bcs_cplx(&b[2], S->zBC.lCode, S->zBC.rCode, S->zBC.lVal_r, S->zBC.lVal_i, S->zBC.rVal_r, S->zBC.rVal_i);
+      break;
+    }
+    case DOUBLE_COMPLEX: {
+      const multi_UBspline_3d_z *S = &h->pub.z;
+      g[0] = S->x_grid; g[1] = S->y_grid; g[2] = S->z_grid;
+      bcs_cplx(&b[0], S->xBC.lCode, S->xBC.rCode, S->xBC.lVal_r, S->xBC.lVal_i, S->xBC.rVal_r, S->xBC.rVal_i);
+      bcs_cplx(&b[1], S->yBC.lCode, S->yBC.rCode, S->yBC.lVal_r, S->yBC.lVal_i, S->yBC.rVal_r, S->yBC.rVal_i);
+      bcs_cplx(&b[2], S->zBC.lCode, S->zBC.rCode, S->zBC.lVal_r, S->zBC.lVal_i, S->zBC.rVal_r, S->zBC.rVal_i);
+      break;
+    }
+    default:
+      return -1;
+  }
+  memset(ax, 0, 3 * sizeof(*ax));
+  for (int a = 0; a < 3; ++a) {
+    axis_op op;
+    const int rc = axis_init(&op, &g[a], b[a].l, b[a].r);
+    if (rc) {
+      for (int k = 0; k < a; ++k) free(ax[k].arrays);
+      return rc;
+    }
+    const int M = op.M;
+    double *arr = calloc((size_t)4 * M, sizeof(double));
+    if (!arr) {
+      axis_free(&op);
+      for (int k = 0; k < a; ++k) free(ax[k].arrays);
+      return -2;
+    }
+    const int m = op.kind == 2 ? M : M - 1;       /* rows of the factorised core */
+    memcpy(arr, op.inv, sizeof(double) * m);
+    memcpy(arr + M, op.g, sizeof(double) * m);
+    memcpy(arr + 2 * M, op.sub, sizeof(double) * m);
+    if (op.w) memcpy(arr + 3 * M, op.w, sizeof(double) * m);
+    ax[a].M = M; ax[a].n = op.n; ax[a].kind = op.kind;
+    ax[a].sgn = op.sgn; ax[a].zinv = op.zinv;
+    ax[a].l0 = op.l0; ax[a].r2 = op.r2;
+    memcpy(ax[a].lv, b[a].lv, sizeof b[a].lv);
+    memcpy(ax[a].rv, b[a].rv, sizeof b[a].rv);
+    ax[a].arrays = arr;
+    axis_free(&op);
+  }
+  return 0;
+}
 
 /* ------------------------------------------------------------------------ */
 

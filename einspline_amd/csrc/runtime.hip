@@ -21,6 +21,7 @@
 #include "einspline_private.h"
 #include "launch.hpp"
 #include "runtime_internal.hpp"
+#include "convert.hpp"
 
 using esk::Launch;
 using esk::Params;
@@ -62,31 +63,6 @@ static size_t storage_bytes(einspline_storage s) {
 
 namespace {
 
-// double -> float rounded to odd: exact when representable, otherwise the
-// truncated value with its last bit set.  Rounding that result to a format
-// with at least two fewer mantissa bits (bf16, f16) equals rounding the
-// double directly.
-__device__ __forceinline__ float f64_to_f32_odd(double d) {
-  float f = (float)d;
-  if ((double)f == d || f != f) return f;
-  unsigned u = __float_as_uint(f);
-  if (fabs((double)f) > fabs(d)) u -= 1;        // step the magnitude back toward zero
-  return __uint_as_float(u | 1u);
-}
-
-__device__ __forceinline__ uint16_t f32_to_bf16(float f, int &ovf) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);   // NaN stays NaN
-  u += 0x7fffu + ((u >> 16) & 1u);
-  if ((u & 0x7f800000u) == 0x7f800000u) ovf = 1;                             // rounded to inf
-  return (uint16_t)(u >> 16);
-}
-
-__device__ __forceinline__ uint16_t f32_to_f16(float f, int &ovf) {
-  if (fabsf(f) >= 65520.0f) ovf = 1;            // beyond the largest half, rounds to inf
-  return __builtin_bit_cast(uint16_t, (_Float16)f);
-}
-
 // Copy `nrows` rows of `nreal` source reals (float or double) into table
 // rows of `row` storage elements starting at row r0, zero-padding each row.
 template <class Src>
@@ -104,10 +80,10 @@ __global__ void __launch_bounds__(256) convert_rows(const Src *__restrict__ src,
       case EINSPLINE_STORE_F32: ((float *)table)[o] = (float)v; break;
       case EINSPLINE_STORE_F64: ((double *)table)[o] = v; break;
       case EINSPLINE_STORE_BF16:
-        ((uint16_t *)table)[o] = f32_to_bf16(sizeof(Src) == 8 ? f64_to_f32_odd(v) : (float)v, ovf);
+        ((uint16_t *)table)[o] = esk::f32_to_bf16(sizeof(Src) == 8 ? esk::f64_to_f32_odd(v) : (float)v, ovf);
         break;
       case EINSPLINE_STORE_F16:
-        ((uint16_t *)table)[o] = f32_to_f16(sizeof(Src) == 8 ? f64_to_f32_odd(v) : (float)v, ovf);
+        ((uint16_t *)table)[o] = esk::f32_to_f16(sizeof(Src) == 8 ? esk::f64_to_f32_odd(v) : (float)v, ovf);
         break;
     }
   }

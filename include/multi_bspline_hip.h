@@ -289,6 +289,24 @@ int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_point
 void einspline_hip_invalidate(void *spline);
 
 /* ---- minimal device runtime (so C and ctypes callers need no HIP) ------ */
+/* ---- tables solved on the GPU ------------------------------------------ */
+/* Build a 3-D device table from samples already in device memory: the solve
+ * of set_multi_UBspline_3d_* (src/multi_bspline_create.c:323-1477) runs on
+ * the GPU, so neither the host solve nor a coefficient upload is needed.
+ *   spline   a 3-D host spline from create_multi_UBspline_3d_{s,d,c,z}: its
+ *            grids, boundary conditions and num_splines (its host coefficients
+ *            are neither read nor written);
+ *   data_d   num_splines blocks of Mx*My*Mz samples, in the order successive
+ *            set_multi_UBspline_3d_* calls take them (float for s/c, double
+ *            for d/z; complex samples as re,im pairs);
+ *   storage  -1 for the spline's own type (s/c F32, d/z F64); a d spline may
+ *            also ask for F32, BF16 or F16 (as ..._s_hip_store) and a z spline
+ *            for F32 (as ..._c_hip_conv).
+ * The solve is done in double, with the host's operations in the host's
+ * order, so the table equals the set + upload one bit for bit.  Runs on
+ * `stream` and waits for it.  NULL on failure (einspline_hip_last_error). */
+multi_UBspline_3d_hip *einspline_build_table_hip(void *spline, const void *data_d, int storage, void *stream);
+
 /* ---- tables on disk (no reference counterpart: einspline has no table file) */
 /* Save a device table of any dimension and storage to `path`, as the kernels
  * read it: a 512-byte header (grids, sizes, storage type, tuned tile, point
