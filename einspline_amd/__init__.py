@@ -241,6 +241,67 @@ _proto("einspline_hip_stream_copy", _i, _vp, _vp, _sz, _vp)
 _proto("einspline_hip_stream_fill", _i, _vp, C.c_uint, _sz, _i, _vp)
 _proto("einspline_table_save", _i, C.POINTER(multi_UBspline_3d_hip), C.c_char_p)
 _proto("einspline_build_table_hip", C.POINTER(multi_UBspline_3d_hip), _vp, _vp, _i, _vp)
+
+
+# ---------------------------------------------------------------------------
+# non-uniform grids and 1-D non-uniform multi-splines (include/nugrid.h,
+# include/nubasis.h, include/multi_nubspline.h)
+
+GENERAL_GRID, CENTER_GRID, LOG_GRID = 1, 2, 3
+
+
+class NUgrid(C.Structure):
+    _fields_ = [("code", C.c_int), ("start", C.c_double), ("end", C.c_double),
+                ("points", C.POINTER(C.c_double)), ("num_points", C.c_int), ("reverse_map", C.c_void_p)]
+
+
+class NUBasis(C.Structure):
+    _fields_ = [("grid", C.POINTER(NUgrid)), ("xVals", C.POINTER(C.c_double)),
+                ("dxInv", C.POINTER(C.c_double)), ("periodic", C.c_bool)]
+
+
+def _nub(bc):
+    return [("spcode", C.c_int), ("tcode", C.c_int), ("coefs", C.c_void_p), ("x_stride", C.c_ssize_t),
+            ("xBC", bc), ("num_splines", C.c_int), ("x_grid", C.POINTER(NUgrid)),
+            ("x_basis", C.POINTER(NUBasis))]
+
+
+class multi_NUBspline_1d_d(C.Structure):
+    _fields_ = _nub(BCtype_d)
+
+
+class multi_NUBspline_1d_z(C.Structure):
+    _fields_ = _nub(BCtype_z)
+
+
+_GP, _BP = C.POINTER(NUgrid), C.POINTER(NUBasis)
+_proto("create_general_grid", _GP, _vp, _i)
+_proto("create_center_grid", _GP, _d, _d, _d, _i)
+_proto("create_log_grid", _GP, _d, _d, _i)
+_proto("destroy_grid", None, _GP)
+_proto("create_NUBasis", _BP, _GP, C.c_bool)
+_proto("destroy_NUBasis", None, _BP)
+for _t in "sd":
+    _proto(f"get_NUBasis_funcs_{_t}", _i, _BP, _d, _vp)
+    _proto(f"get_NUBasis_funcs_{_t}i", None, _BP, _i, _vp)
+    _proto(f"get_NUBasis_dfuncs_{_t}", _i, _BP, _d, _vp, _vp)
+    _proto(f"get_NUBasis_dfuncs_{_t}i", None, _BP, _i, _vp, _vp)
+    _proto(f"get_NUBasis_d2funcs_{_t}", _i, _BP, _d, _vp, _vp, _vp)
+    _proto(f"get_NUBasis_d2funcs_{_t}i", None, _BP, _i, _vp, _vp, _vp)
+_proto("create_multi_NUBspline_1d_d", C.POINTER(multi_NUBspline_1d_d), _GP, BCtype_d, _i)
+_proto("set_multi_NUBspline_1d_d", None, C.POINTER(multi_NUBspline_1d_d), _i, _vp)
+_proto("set_multi_NUBspline_1d_d_BC", None, C.POINTER(multi_NUBspline_1d_d), _i, _vp, BCtype_d)
+_proto("create_multi_NUBspline_1d_z", C.POINTER(multi_NUBspline_1d_z), _GP, BCtype_z, _i)
+_proto("set_multi_NUBspline_1d_z", None, C.POINTER(multi_NUBspline_1d_z), _i, _vp)
+_proto("set_multi_NUBspline_1d_z_BC", None, C.POINTER(multi_NUBspline_1d_z), _i, _vp, BCtype_z)
+_proto("destroy_multi_NUBspline", None, _vp)
+_proto("eval_multi_NUBspline_1d_z", None, C.POINTER(multi_NUBspline_1d_z), _d, _vp)
+_proto("eval_multi_NUBspline_1d_z_vg", None, C.POINTER(multi_NUBspline_1d_z), _d, _vp, _vp)
+_proto("eval_multi_NUBspline_1d_z_vgl", None, C.POINTER(multi_NUBspline_1d_z), _d, _vp, _vp, _vp)
+_proto("eval_multi_NUBspline_1d_z_vgh", None, C.POINTER(multi_NUBspline_1d_z), _d, _vp, _vp, _vp)
+_proto("create_multi_NUBspline_1d_z_hip", _vp, C.POINTER(multi_NUBspline_1d_z))
+_proto("destroy_multi_NUBspline_1d_hip", None, _vp)
+_proto("einspline_eval_nub_1d_z_batch", _i, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp)
 _proto("einspline_table_load", C.POINTER(multi_UBspline_3d_hip), C.c_char_p)
 # Slater-determinant kernels (include/determinant_update_hip.h)
 for _k in "sd":
@@ -617,3 +678,83 @@ class DeviceTable:
             self.destroy()
         except Exception:
             pass
+
+
+class NUGrid:
+    """A non-uniform grid made by the library: NUGrid.general(points),
+    NUGrid.center(start, end, ratio, n) or NUGrid.log(start, end, n)."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            raise EinsplineError(E_INVAL, "create grid")
+        self.ptr = ptr
+
+    @classmethod
+    def general(cls, points):
+        p = np.ascontiguousarray(points, np.float64)
+        return cls(lib.create_general_grid(p.ctypes.data, len(p)))
+
+    @classmethod
+    def center(cls, start, end, ratio, n):
+        return cls(lib.create_center_grid(float(start), float(end), float(ratio), int(n)))
+
+    @classmethod
+    def log(cls, start, end, n):
+        return cls(lib.create_log_grid(float(start), float(end), int(n)))
+
+    @property
+    def points(self):
+        g = self.ptr.contents
+        return np.ctypeslib.as_array(g.points, (g.num_points,)).copy()
+
+    def reverse(self, x):
+        g = self.ptr.contents
+        f = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double)(g.reverse_map)
+        return f(C.cast(self.ptr, C.c_void_p), float(x))
+
+    def destroy(self):
+        if self.ptr:
+            lib.destroy_grid(self.ptr)
+            self.ptr = None
+
+
+class NUBSpline1D:
+    """A multi_NUBspline_1d_{d,z}: kind 'd' or 'z', a NUGrid (which must
+    outlive it), bc = (lcode, rcode[, lval, rval]), N orbitals."""
+
+    def __init__(self, kind, grid: NUGrid, bc, N):
+        assert kind in "dz"
+        self.kind, self.grid, self.N = kind, grid, int(N)
+        lc, rc, lv, rv = tuple(bc) + (0.0,) * (4 - len(bc))
+        if kind == "d":
+            b = BCtype_d(lc, rc, float(lv), float(rv))
+            self.ptr = lib.create_multi_NUBspline_1d_d(grid.ptr, b, self.N)
+        else:
+            lv, rv = complex(lv), complex(rv)
+            b = BCtype_z(lc, rc, lv.real, lv.imag, rv.real, rv.imag)
+            self.ptr = lib.create_multi_NUBspline_1d_z(grid.ptr, b, self.N)
+        if not self.ptr:
+            raise EinsplineError(E_INVAL, f"create_multi_NUBspline_1d_{kind}")
+        self.dtype = np.complex128 if kind == "z" else np.float64
+
+    def set(self, n, data):
+        d = np.ascontiguousarray(data, self.dtype)
+        getattr(lib, f"set_multi_NUBspline_1d_{self.kind}")(self.ptr, int(n), d.ctypes.data)
+
+    def coefs(self):
+        t = self.ptr.contents
+        rows = t.x_grid.contents.num_points + 2
+        buf = (C.c_char * (rows * self.N * np.dtype(self.dtype).itemsize)).from_address(t.coefs)
+        return np.frombuffer(buf, self.dtype).reshape(rows, self.N).copy()
+
+    def eval(self, mode, x):
+        """Single-point evaluation (z only): 'v', 'vg', 'vgl' or 'vgh'."""
+        outs = [np.zeros(self.N, np.complex128) for _ in range({"v": 1, "vg": 2}.get(mode, 3))]
+        suf = "" if mode == "v" else "_" + mode
+        getattr(lib, f"eval_multi_NUBspline_1d_z{suf}")(self.ptr, float(x), *[o.ctypes.data for o in outs])
+        return tuple(outs)
+
+    def destroy(self):
+        if self.ptr:
+            lib.destroy_multi_NUBspline(self.ptr)
+            self.ptr = None

@@ -6,6 +6,7 @@
 #include <pthread.h>
 #include <stdint.h>
 #include "multi_bspline_structs.h"
+#include "multi_nubspline.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -60,6 +61,27 @@ typedef struct {
 /* Fill ax[0..2] for a 3-D host spline.  Returns 0, -1 (not 3-D or an
  * unsupported boundary pair) or -2 (out of memory). */
 int einspline_export_axes(void *spline, einspline_axis_export ax[3]);
+
+/* Non-uniform 1-D multi-splines (nub_create.c): the public struct, then
+ * this tail.  `mirror` is the device copy (multi_NUBspline_1d_hip*) used by
+ * the single-point evaluation; set_* marks it stale. */
+#define EINSPLINE_NUB_MAGIC 0x4e554231u /* "NUB1" */
+typedef struct einspline_nub1d {
+  union {
+    multi_NUBspline_1d_d d;
+    multi_NUBspline_1d_z z;
+  } pub;
+  uint32_t magic;
+  int ncomp;               /* 1 (d) or 2 (z) */
+  int nrows;               /* coefficient rows: num_points + 2 */
+  int dirty;
+  void *mirror;
+  pthread_mutex_t lock;
+} einspline_nub1d;
+
+einspline_nub1d *einspline_nub_of(void *spline);
+/* Implemented by the HIP runtime: releases a non-uniform device mirror. */
+void einspline_nub_mirror_release(void *mirror);
 
 /* Implemented by the HIP runtime: releases a device mirror. */
 void einspline_mirror_release(void *mirror);
