@@ -58,6 +58,7 @@ MULTI_U1D, MULTI_U2D, MULTI_U3D = 6, 7, 8
 STORE_F32, STORE_F64, STORE_BF16, STORE_F16 = range(4)
 MODES = ("v", "vg", "vgl", "vgh")          # autotuner modes 0..3; "vghgh" is batch-only
 ORDER_GIVEN, ORDER_LOCALITY = 0, 1
+KERNEL_VECTOR, KERNEL_MATRIX = 0, 1
 
 E_OK, E_INVAL, E_NOMEM, E_HIP, E_RANGE, E_BUDGET, E_BUSY = 0, -1, -2, -3, -4, -5, -6
 
@@ -214,6 +215,8 @@ _proto("einspline_autotune", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i, C.POI
        C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_float))
 _proto("einspline_set_store_policy", _i, C.POINTER(multi_UBspline_3d_hip), _i)
 _proto("einspline_set_point_order", _i, C.POINTER(multi_UBspline_3d_hip), _i)
+_proto("einspline_set_kernel", _i, C.POINTER(multi_UBspline_3d_hip), _i)
+_proto("einspline_get_kernel", _i, C.POINTER(multi_UBspline_3d_hip))
 _proto("einspline_get_tuning", _i, C.POINTER(multi_UBspline_3d_hip), C.POINTER(_i), C.POINTER(_i),
        C.POINTER(_i), C.POINTER(_i))
 _proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp)
@@ -613,6 +616,17 @@ class DeviceTable:
         """ORDER_GIVEN evaluates points as passed; ORDER_LOCALITY sorts each
         batch of >= 1024 points by cell first (same results, better reuse)."""
         check(lib.einspline_set_point_order(self.ptr, int(order)), "set_point_order")
+
+    def set_kernel(self, kernel=KERNEL_VECTOR):
+        """KERNEL_MATRIX evaluates vgh of a BF16/FP16 table (strided batches)
+        on the matrix cores; KERNEL_VECTOR (default) uses the vector kernel."""
+        check(lib.einspline_set_kernel(self.ptr, int(kernel)), "set_kernel")
+
+    @property
+    def kernel(self):
+        k = lib.einspline_get_kernel(self.ptr)
+        check(min(k, 0), "get_kernel")
+        return k
 
     def tuning(self):
         """(lane_bytes, block, store_policy, point_order) in effect."""

@@ -427,6 +427,9 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   const S *base = (const S *)P.coefs + (long)ix * P.xs + (long)iy * P.ys + (long)iz * P.zs +
                   (long)ch * V;
   const long zs = P.zs;
+  auto load_row = [&](int nx, int k, A *dst) {
+    L::load(base + (nx >> 2) * P.xs + (nx & 3) * P.ys + k * zs, dst);
+  };
 
   // 16 (i,j) steps, fully unrolled so the basis indices are constants.  The
   // rows of step st+PF are requested while step st is reduced, and a
@@ -436,9 +439,8 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   A rb[PF + 1][4][V];
 #pragma unroll
   for (int st = 0; st < PF; ++st) {
-    const S *rp = base + (st >> 2) * P.xs + (st & 3) * P.ys;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L::load(rp + k * zs, rb[st][k]);
+    for (int k = 0; k < 4; ++k) load_row(st, k, rb[st][k]);
   }
 #if ESK_YX
   // y partials per x row: b s0, db s0, d2b s0, b s1, db s1, b s2 (+ d3b s0,
@@ -451,9 +453,8 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
     const int i = st >> 2, j = st & 3;
     if (st + PF < 16) {
       const int nx = st + PF;
-      const S *rp = base + (nx >> 2) * P.xs + (nx & 3) * P.ys;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) L::load(rp + k * zs, rb[nx % (PF + 1)][k]);
+      for (int k = 0; k < 4; ++k) load_row(nx, k, rb[nx % (PF + 1)][k]);
     }
     A (&r)[4][V] = rb[st % (PF + 1)];
 

@@ -15,7 +15,19 @@ struct Launch {
   int api;           // Api
   int lane_bytes;    // 8, 16 or 32 (16-bit tables also 8..32)
   int block;         // work-items per block, multiple of 64, <= 256
+  int kernel;        // 16-bit tables: 0 vector kernel, 1 matrix-core kernel (vgh, strided)
 };
+
+// vgh of a 16-bit table on the matrix cores (mfma_kernel.hpp); `f16` picks
+// FP16 storage, else BF16.
+hipError_t launch_mfma16(const Launch &L, const Params &P, bool f16, hipStream_t s);
+
+// The matrix-core kernel serves strided vgh of real 16-bit tables;
+// everything else runs on the vector kernel.
+inline bool use_mfma16(const Launch &L, const Params &P) {
+  (void)P;
+  return L.kernel == 1 && L.mode == MODE_VGH && L.api == API_STRIDED && !L.cplx;
+}
 
 hipError_t launch_f32(const Launch &L, const Params &P, hipStream_t s);
 hipError_t launch_f64(const Launch &L, const Params &P, hipStream_t s);

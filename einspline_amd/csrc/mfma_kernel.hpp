@@ -1,0 +1,304 @@
+// mfma_kernel.hpp -- vgh of 16-bit (BF16 / FP16) tables on the matrix cores.
+//
+// For one point the outputs of every orbital are a small matrix product:
+//   D[q][n] = sum_K W[q][K] P[K][n],   K < 64 (the stencil rows,
+//   K = 16 i + 4 j + k), n < orbitals,
+// where W[q][K] = X_q(i) Y_q(j) Z_q(k) is a product of basis values and
+// P[K] is stencil row K.  The rows q are the point's outputs in the order
+// they are stored per orbital: v, the 3 gradient entries, then the Hessian
+// as stored (all 9 entries row-major, or the 6 of the packed form), so 13
+// or 10 of the 16 rows of a tile are used; the duplicated Hessian rows cost
+// nothing extra.  One wavefront takes one point and runs it as 16 x 16 x 32
+// MFMA tiles: W is the A operand, the stencil rows are the B operand,
+// accumulation is FP32.
+//
+// Operand layout of v_mfma_f32_16x16x32_{bf16,f16} (lane l, g = l / 16,
+// r = l % 16): A holds W[r][8g .. 8g+7], B holds P[8g .. 8g+7][column r], D
+// holds D[4g .. 4g+3][column r].  A lane's B column must therefore be eight
+// stencil rows of one orbital, while memory holds rows of consecutive
+// orbitals.  Lane (g, r) loads eight rows (8g .. 8g+7 of the 32-row chunk),
+// 16 bytes each at orbitals 8r .. 8r+7 of a 128-orbital slab: four lane
+// groups read four row blocks of 256 contiguous bytes.  Tile t (0..7) then
+// takes column r to be orbital 8r + t, and the lane builds its B fragment
+// from element t of each of its eight rows with byte permutes: an in-lane
+// 8 x 8 transpose instead of an exchange between lanes.
+//
+// The weights are split into a 16-bit high part and a 16-bit remainder and
+// both are applied (two MFMAs per tile), so they carry ~16 mantissa bits;
+// the result's error is the table's own 16-bit rounding, as on the vector
+// path.
+//
+// Results leave through LDS: lane (g, r) holds rows 4g .. 4g+3 of its
+// orbitals, and writes them to the wavefront's staging area at their places
+// in the output layout; the wavefront then copies the slab's v, gradient and
+// Hessian blocks out as contiguous runs (16-byte stores where aligned).
+#pragma once
+#include <type_traits>
+#include "eval_kernel.hpp"
+
+namespace esk {
+
+template <class S> struct Mfma16;
+template <> struct Mfma16<bf16s> {
+  typedef __bf16 h8 __attribute__((ext_vector_type(8)));
+  __device__ static __forceinline__ __bf16 narrow(float f) { return (__bf16)f; }
+  __device__ static __forceinline__ float back(__bf16 h) { return (float)h; }
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __device__ static __forceinline__ f4 mma(h8 a, h8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma16<f16s> {
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  __device__ static __forceinline__ _Float16 narrow(float f) { return (_Float16)f; }
+  __device__ static __forceinline__ float back(_Float16 h) { return (float)h; }
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __device__ static __forceinline__ f4 mma(h8 a, h8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+// Pack the 16-bit halves `t & 1` of x and y into one dword (x low, y high).
+__device__ __forceinline__ uint32_t pick16(uint32_t x, uint32_t y, int hi) {
+  return __builtin_amdgcn_perm(y, x, hi ? 0x07060302u : 0x05040100u);
+}
+
+// Derivative orders (x, y, z) of output row q, two bits each, and the row
+// count: full Hessian rows v gx gy gz xx xy xz yx yy yz zx zy zz; packed rows
+// v gx gy gz xx xy xz yy yz zz.
+template <bool PACKED> struct MfmaRows;
+template <> struct MfmaRows<false> {
+  static constexpr int NQ = 13, HS = 9;
+  static constexpr uint32_t OX = 0u | 1u << 2 | 2u << 8 | 1u << 10 | 1u << 12 | 1u << 14 | 1u << 20;
+  static constexpr uint32_t OY = 1u << 4 | 1u << 10 | 1u << 14 | 2u << 16 | 1u << 18 | 1u << 22;
+  static constexpr uint32_t OZ = 1u << 6 | 1u << 12 | 1u << 18 | 1u << 20 | 1u << 22 | 2u << 24;
+};
+template <> struct MfmaRows<true> {
+  static constexpr int NQ = 10, HS = 6;
+  static constexpr uint32_t OX = 1u << 2 | 2u << 8 | 1u << 10 | 1u << 12;
+  static constexpr uint32_t OY = 1u << 4 | 1u << 10 | 2u << 14 | 1u << 16;
+  static constexpr uint32_t OZ = 1u << 6 | 1u << 12 | 1u << 16 | 2u << 18;
+};
+
+// Floats of one wavefront's staging area: v, gradient and Hessian blocks of a
+// 128-orbital slab in output layout (the full Hessian, the larger case).
+constexpr int kMfmaStage = 128 * (1 + 3 + 9);
+
+// Register budget: three wavefronts per SIMD (at most 168 VGPRs; the
+// kernel needs about 150 without spills, while four would spill).
+template <class S, bool PACKED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+eval3d_mfma(const Params P) {
+  using M = Mfma16<S>;
+  using Rows = MfmaRows<PACKED>;
+  typedef typename M::h8 h8;
+  typedef typename M::f4 f4;
+  __shared__ __align__(16) float stage_all[4 * kMfmaStage];
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, r = lane & 15;
+  float *const stage = stage_all + (threadIdx.x >> 6) * kMfmaStage;
+  unsigned b = blockIdx.x;
+  if (P.xcd_runs) {
+    const unsigned nb = gridDim.x, qq = nb >> 3, rr = nb & 7u, x = b & 7u, l = b >> 3;
+    b = (x < rr ? x * (qq + 1) : rr * (qq + 1) + (x - rr) * qq) + l;
+  }
+  int p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (p >= P.npoints) return;
+  if (P.perm) p = uniform(P.perm[p]);
+
+  const double *pp = (const double *)P.pos + 3l * p;
+  int ix, iy, iz;
+  float tx, ty, tz;
+  locate<float>(pp[0], P.x0, P.xi, P.nx4, ix, tx);
+  locate<float>(pp[1], P.y0, P.yi, P.ny4, iy, ty);
+  locate<float>(pp[2], P.z0, P.zi, P.nz4, iz, tz);
+  ix = uniform(ix); iy = uniform(iy); iz = uniform(iz);
+  tx = uniform(tx); ty = uniform(ty); tz = uniform(tz);
+  float wx[3][4], wy[3][4], wz[3][4];
+  basis0(tx, wx[0]); basis1(tx, wx[1]); basis2(tx, wx[2]);
+  basis0(ty, wy[0]); basis1(ty, wy[1]); basis2(ty, wy[2]);
+  basis0(tz, wz[0]); basis1(tz, wz[1]); basis2(tz, wz[2]);
+
+  // FP16 holds magnitudes up to 65504 only: points outside the grid box
+  // extrapolate with weights that grow like t^3 per axis, so products bounded
+  // above 1024 are scaled by a power of two (exact) that brings the largest
+  // below 1, and the outputs by its inverse.  Inside the box the bound is at
+  // most 8 (no scaling); BF16 has the FP32 range.
+  float wscale = 1.0f, oscale = 1.0f;
+  if constexpr (std::is_same<S, f16s>::value) {
+    float mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        mx = fmaxf(mx, fabsf(wx[o][u]));
+        my = fmaxf(my, fabsf(wy[o][u]));
+        mz = fmaxf(mz, fabsf(wz[o][u]));
+      }
+    const float bound = mx * my * mz;         // >= |W[q][K]| for every q, K
+    if (bound > 1024.0f) {
+      int e;
+      (void)frexpf(bound, &e);                // bound < 2^e
+      wscale = ldexpf(1.0f, -e);
+      oscale = ldexpf(1.0f, e);
+    }
+  }
+
+  // A operands (W split into high and low 16-bit parts) for the two K chunks.
+  // Lane (g, r): row q = r, columns K = 32 c + 8 g + jj, i.e. x row
+  // i = 2c + g/2, y row j = 2 (g & 1) + jj/4, z row k = jj & 3.  Every
+  // selection below is a compare-and-select: no register array is indexed
+  // with a lane-dependent value.
+  const int q = r;
+  const int ox = (Rows::OX >> (2 * q)) & 3, oy = (Rows::OY >> (2 * q)) & 3, oz = (Rows::OZ >> (2 * q)) & 3;
+  // (choices are made between values, never between array elements: a
+  // conditional over two lvalues selects an address, and the array then
+  // lives in scratch memory)
+  auto by_order = [](int o, float w0, float w1, float w2) { return o == 0 ? w0 : o == 1 ? w1 : w2; };
+  const bool gx1 = (g >> 1) != 0, gy1 = (g & 1) != 0;
+#define ESK_PICK(o, w, u) by_order(o, w[0][u], w[1][u], w[2][u])
+  const float x0 = ESK_PICK(ox, wx, 0), x1 = ESK_PICK(ox, wx, 1), x2 = ESK_PICK(ox, wx, 2), x3 = ESK_PICK(ox, wx, 3);
+  const float y0 = ESK_PICK(oy, wy, 0), y1 = ESK_PICK(oy, wy, 1), y2 = ESK_PICK(oy, wy, 2), y3 = ESK_PICK(oy, wy, 3);
+  const float z0 = ESK_PICK(oz, wz, 0), z1 = ESK_PICK(oz, wz, 1), z2 = ESK_PICK(oz, wz, 2), z3 = ESK_PICK(oz, wz, 3);
+#undef ESK_PICK
+  const float xc0 = gx1 ? x1 : x0, xc1 = gx1 ? x3 : x2;     // x row 2c + g/2 for c = 0, 1
+  const float yl = gy1 ? y2 : y0, yh = gy1 ? y3 : y1;       // y row 2 (g & 1) + jj/4
+  h8 ahi[2], alo[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const float xi = c == 0 ? xc0 : xc1;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const float yj = jj < 4 ? yl : yh;
+      const int kk = jj & 3;
+      const float zk = kk == 0 ? z0 : kk == 1 ? z1 : kk == 2 ? z2 : z3;
+      const float w = q < Rows::NQ ? xi * yj * zk * wscale : 0.0f;
+      const auto h = M::narrow(w);
+      ahi[c][jj] = h;
+      alo[c][jj] = M::narrow(w - M::back(h));
+    }
+  }
+
+  // Accumulator entry e of lane (g, r) is output row 4g + e of orbital
+  // 8r + t (tile t) of the slab; its output scale (derivatives in physical
+  // units):
+  const float dxi = (float)P.xi, dyi = (float)P.yi, dzi = (float)P.zi;
+  float sc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int qe = 4 * g + e;
+    const int ex = (Rows::OX >> (2 * qe)) & 3, ey = (Rows::OY >> (2 * qe)) & 3, ez = (Rows::OZ >> (2 * qe)) & 3;
+    float f = oscale;
+    f = ex >= 1 ? f * dxi : f; f = ex == 2 ? f * dxi : f;
+    f = ey >= 1 ? f * dyi : f; f = ey == 2 ? f * dyi : f;
+    f = ez >= 1 ? f * dzi : f; f = ez == 2 ? f * dzi : f;
+    sc[e] = f;
+  }
+  // The staging area holds the slab in output layout: v block (128 floats),
+  // gradient block (384), Hessian block (128 HS).  Lanes g = 0 hold v and
+  // the gradient of their eight orbitals (8 and 24 consecutive floats),
+  // lanes g > 0 Hessian entries 4 (g - 1) + e at stride HS per orbital; each
+  // group's writes are its base address plus constant offsets.
+  float *const vg_base = stage + 8 * r;                          // v of orbital 8r
+  float *const gr_base = stage + 128 + 24 * r;                   // gradient of orbital 8r
+  float *const h_base = stage + 512 + Rows::HS * 8 * r + 4 * (g - 1);
+
+  const S *cell = (const S *)P.coefs + (long)ix * P.xs + (long)iy * P.ys + (long)iz * P.zs;
+  const int row_reals = (int)P.zs;
+  const int end = min(row_reals, P.nreal);            // slabs start below this
+  const bool nt = P.store == STORE_STAGED_NT;
+  float *ov = (float *)P.out_v + (long)p * P.ld_v;
+  float *og = (float *)P.out_g + (long)p * P.ld_g;
+  float *oh = (float *)P.out_h + (long)p * P.ld_h;
+
+  // Eight stencil rows (K = 32 c + 8 g + jj) of the lane's eight orbitals
+  // from o0: one lane offset (x row g/2, y rows 2 (g & 1) + .., orbital o0)
+  // plus a wave-uniform offset per row.  Rows are padded to whole 32-byte
+  // pieces, so a lane whose o0 is inside the row reads inside it; other
+  // lanes read zeros.  (Buffer loads with the same offsets ran 30% slower
+  // in the locality order on MI355X.)
+  constexpr int EB = (int)sizeof(S);
+  const long lane_off = ((g >> 1) * P.xs + 2 * (g & 1) * P.ys) * EB;
+  auto load_half = [&](uint4 (&R)[8], int s0, int c) {
+    const int o0 = s0 + 8 * r;
+    const bool have = o0 < row_reals;
+    const char *base = (const char *)cell + lane_off + (long)o0 * EB;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const long soff = (2 * c * P.xs + (jj >> 2) * P.ys + (jj & 3) * P.zs) * EB;
+      R[jj] = have ? *(const uint4 *)(base + soff) : uint4{0u, 0u, 0u, 0u};
+    }
+  };
+  // Tile t takes element t of each row (an in-lane 8 x 8 transpose).
+  auto mma_half = [&](const uint4 (&R)[8], f4 (&acc)[8], int c) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int d = t >> 1, hi = t & 1;
+      uint32_t w4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t x = d == 0 ? R[2 * u].x : d == 1 ? R[2 * u].y : d == 2 ? R[2 * u].z : R[2 * u].w;
+        const uint32_t y = d == 0 ? R[2 * u + 1].x : d == 1 ? R[2 * u + 1].y : d == 2 ? R[2 * u + 1].z : R[2 * u + 1].w;
+        w4[u] = pick16(x, y, hi);
+      }
+      h8 bf;
+      __builtin_memcpy(&bf, w4, sizeof bf);
+      acc[t] = M::mma(c == 0 ? ahi[0] : ahi[1], bf, acc[t]);
+      acc[t] = M::mma(c == 0 ? alo[0] : alo[1], bf, acc[t]);
+    }
+  };
+  auto copy_out = [&](const float *src, float *dst, int n) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    int done = 0;
+    if (((uintptr_t)dst & 15) == 0) {
+      const int n4 = n >> 2;
+      for (int i = lane; i < n4; i += 64) st_out(*(const v4 *)(src + 4 * i), (v4 *)(dst + 4 * i), nt);
+      done = 4 * n4;
+    }
+    for (int i = done + lane; i < n; i += 64) dst[i] = src[i];
+  };
+
+  // Software pipeline over half-slabs: the rows of the next half are in
+  // flight while the current one is multiplied and the slab is stored.
+  uint4 R0[8], R1[8];
+  load_half(R0, 0, 0);
+  for (int s0 = 0; s0 < end; s0 += 128) {
+    f4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    load_half(R1, s0, 1);
+    mma_half(R0, acc, 0);
+    if (s0 + 128 < end) load_half(R0, s0 + 128, 0);
+    mma_half(R1, acc, 1);
+    // stage the slab (orbitals past the valid ones are staged but not
+    // copied out)
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    if (g == 0) {
+      *(v4 *)vg_base = v4{acc[0][0], acc[1][0], acc[2][0], acc[3][0]} * sc[0];
+      *(v4 *)(vg_base + 4) = v4{acc[4][0], acc[5][0], acc[6][0], acc[7][0]} * sc[0];
+#pragma unroll
+      for (int t = 0; t < 8; t += 4) {
+        const v4 a = {acc[t][1] * sc[1], acc[t][2] * sc[2], acc[t][3] * sc[3], acc[t + 1][1] * sc[1]};
+        const v4 b = {acc[t + 1][2] * sc[2], acc[t + 1][3] * sc[3], acc[t + 2][1] * sc[1], acc[t + 2][2] * sc[2]};
+        const v4 c = {acc[t + 2][3] * sc[3], acc[t + 3][1] * sc[1], acc[t + 3][2] * sc[2], acc[t + 3][3] * sc[3]};
+        *(v4 *)(gr_base + 3 * t) = a;
+        *(v4 *)(gr_base + 3 * t + 4) = b;
+        *(v4 *)(gr_base + 3 * t + 8) = c;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (4 * g + e < Rows::NQ) h_base[Rows::HS * t + e] = acc[t][e] * sc[e];
+    }
+    wave_sync();
+    const int m = min(128, P.nreal - s0);             // valid orbitals of the slab
+    copy_out(stage, ov + s0, m);
+    copy_out(stage + 128, og + 3 * s0, 3 * m);
+    copy_out(stage + 512, oh + Rows::HS * s0, Rows::HS * m);
+    wave_sync();                                      // reads done before the next slab's writes
+  }
+}
+
+}  // namespace esk

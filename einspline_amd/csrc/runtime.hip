@@ -96,6 +96,7 @@ struct TableExt {
   int block;                   // tile: work-items per block (0 = default)
   int store_policy;            // esk::STORE_* for result stores
   int point_order;             // EINSPLINE_ORDER_*
+  int kernel;                  // EINSPLINE_KERNEL_* (16-bit tables, vgh)
   EskOrderWs *order_ws;        // sort workspace of the locality order (lazily made)
 };
 
@@ -382,6 +383,19 @@ extern "C" int einspline_set_point_order(multi_UBspline_3d_hip *t, int order) {
   return EINSPLINE_OK;
 }
 
+extern "C" int einspline_set_kernel(multi_UBspline_3d_hip *t, int kernel) {
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
+  if (kernel != EINSPLINE_KERNEL_VECTOR && kernel != EINSPLINE_KERNEL_MATRIX)
+    return fail(EINSPLINE_EINVAL, "kernel must be EINSPLINE_KERNEL_VECTOR or EINSPLINE_KERNEL_MATRIX");
+  ((TableExt *)t)->kernel = kernel;
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_get_kernel(const multi_UBspline_3d_hip *t) {
+  if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
+  return ((const TableExt *)t)->kernel;
+}
+
 extern "C" int einspline_get_tuning(const multi_UBspline_3d_hip *t, int *lane_bytes, int *block,
                                     int *store_policy, int *point_order) {
   if (!t) return fail(EINSPLINE_EINVAL, "NULL table");
@@ -406,7 +420,7 @@ struct TuneKey {
   }
 };
 struct TuneVal {
-  int lane_bytes, block, store, order;
+  int lane_bytes, block, store, order, kernel;
 };
 std::map<TuneKey, TuneVal> g_tune;
 bool g_tune_loaded = false;
@@ -418,11 +432,17 @@ void tune_load() {
   if (!f) return;
   FILE *fp = fopen(f, "r");
   if (!fp) return;
-  TuneKey k;
-  TuneVal v;
-  while (fscanf(fp, "%d %d %d %d %d %d %d %d %d", &k.storage, &k.tcode, &k.mode, &k.row_bytes,
-                &k.sample, &v.lane_bytes, &v.block, &v.store, &v.order) == 9)
-    g_tune[k] = v;
+  // one entry per line, 10 integers; lines of any other shape (older files
+  // with fewer fields) are skipped
+  char line[256];
+  while (fgets(line, sizeof line, fp)) {
+    TuneKey k;
+    TuneVal v;
+    char extra;
+    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %c", &k.storage, &k.tcode, &k.mode, &k.row_bytes,
+               &k.sample, &v.lane_bytes, &v.block, &v.store, &v.order, &v.kernel, &extra) == 10)
+      g_tune[k] = v;
+  }
   fclose(fp);
 }
 
@@ -432,9 +452,9 @@ void tune_save() {
   FILE *fp = fopen(f, "w");
   if (!fp) return;
   for (auto &kv : g_tune)
-    fprintf(fp, "%d %d %d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
+    fprintf(fp, "%d %d %d %d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
             kv.first.row_bytes, kv.first.sample, kv.second.lane_bytes, kv.second.block,
-            kv.second.store, kv.second.order);
+            kv.second.store, kv.second.order, kv.second.kernel);
   fclose(fp);
 }
 }  // namespace
@@ -459,6 +479,7 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
       einspline_set_tile(t, it->second.lane_bytes, it->second.block);
       einspline_set_store_policy(t, it->second.store);
       einspline_set_point_order(t, it->second.order);
+      einspline_set_kernel(t, it->second.kernel);
       if (lb_out) *lb_out = it->second.lane_bytes;
       if (blk_out) *blk_out = it->second.block;
       if (nt_out) *nt_out = it->second.store;
@@ -493,8 +514,10 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
     rc = hip_fail(e, "autotune setup");
   TableExt *ext = (TableExt *)t;
   const int old_lb = ext->lane_bytes, old_blk = ext->block, old_store = ext->store_policy;
-  const int old_order = ext->point_order;
+  const int old_order = ext->point_order, old_kernel = ext->kernel;
   int best_lb = 16, best_blk = 256, best_nt = esk::STORE_STAGED_NT, best_order = EINSPLINE_ORDER_GIVEN;
+  int best_kernel = EINSPLINE_KERNEL_VECTOR;
+  ext->kernel = EINSPLINE_KERNEL_VECTOR;
   float best = -1.0f, default_ms = -1.0f;
   const int lbs_f64[] = {16, 32}, lbs_f32[] = {8, 16, 32}, lbs_16[] = {8, 16};
   const bool narrow = t->storage == EINSPLINE_STORE_BF16 || t->storage == EINSPLINE_STORE_F16;
@@ -534,10 +557,40 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
       }
     }
   }
+  // 16-bit tables, vgh: the matrix-core kernel in either order, streaming
+  // or ordinary stores (it has no tile)
+  if (narrow && mode == esk::MODE_VGH) {
+    for (int cand = 0; rc == EINSPLINE_OK && cand < 2 * norders; ++cand) {
+      const int order = cand >> 1, nt = cand & 1 ? esk::STORE_STAGED : esk::STORE_STAGED_NT;
+      ext->kernel = EINSPLINE_KERNEL_MATRIX;
+      ext->point_order = order;
+      ext->store_policy = nt;
+      auto launch = [&]() {
+        return run_strided(t, mode, (const double *)dp, sample, dv, dg, dh, 0, st);
+      };
+      if ((rc = launch()) != EINSPLINE_OK) break;
+      float ms = -1.0f;
+      for (int trial = 0; trial < 3 && rc == EINSPLINE_OK; ++trial) {
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 5 && rc == EINSPLINE_OK; ++r) rc = launch();
+        (void)hipEventRecord(e1, st);
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) { rc = hip_fail(e, "autotune"); break; }
+        float t_ms = 0.0f;
+        (void)hipEventElapsedTime(&t_ms, e0, e1);
+        t_ms /= 5.0f;
+        if (ms < 0.0f || t_ms < ms) ms = t_ms;
+      }
+      if (rc == EINSPLINE_OK && (best < 0.0f || ms < best)) {
+        best = ms; best_kernel = EINSPLINE_KERNEL_MATRIX; best_order = order;
+        best_lb = 16; best_blk = 256; best_nt = nt;
+      }
+    }
+  }
   ext->lane_bytes = old_lb;
   ext->block = old_blk;
   ext->store_policy = old_store;
   ext->point_order = old_order;
+  ext->kernel = old_kernel;
   // keep the default (16 B x 256, streaming stores, given order) unless the
   // winner is at least 3% faster
   if (default_ms > 0.0f && best > 0.97f * default_ms) {
@@ -546,6 +599,7 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
     best_blk = 256;
     best_nt = esk::STORE_STAGED_NT;
     best_order = EINSPLINE_ORDER_GIVEN;
+    best_kernel = EINSPLINE_KERNEL_VECTOR;
   }
   if (dp) (void)hipFree(dp);
   if (dv) (void)hipFree(dv);
@@ -558,9 +612,10 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   einspline_set_tile(t, best_lb, best_blk);
   einspline_set_store_policy(t, best_nt);
   einspline_set_point_order(t, best_order);
+  einspline_set_kernel(t, best_kernel);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = {best_lb, best_blk, best_nt, best_order};
+    g_tune[key] = {best_lb, best_blk, best_nt, best_order, best_kernel};
     tune_save();
   }
   if (lb_out) *lb_out = best_lb;
@@ -590,6 +645,7 @@ int esk_prepare(const multi_UBspline_3d_hip *t, int num, Launch &L, Params &P) {
   if (t->storage == EINSPLINE_STORE_F64 && lb < 16) lb = 16;
   L.lane_bytes = lb;
   L.block = x->block ? x->block : 256;
+  L.kernel = x->kernel;
   L.cplx = t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX;
   memset(&P, 0, sizeof P);
   P.coefs = t->coefs;

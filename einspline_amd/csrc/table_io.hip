@@ -242,6 +242,7 @@ extern "C" multi_UBspline_3d_hip *einspline_table_load(const char *path) {
   if (!eb || hd.dim < 1 || hd.dim > 3 || hd.tcode < SINGLE_REAL || hd.tcode > DOUBLE_COMPLEX || hd.Nx < 4 ||
       hd.Ny < 1 || hd.Nz < 1 || hd.num_splines < 1 ||
       (size_t)hd.row_reals < (size_t)hd.num_splines * (cplx ? 2 : 1) ||
+      (hd.row_reals * eb) % 32 != 0 ||      // the kernels read rows in whole 16- and 32-byte pieces
       hd.coef_bytes != (uint64_t)hd.Nx * hd.Ny * hd.Nz * hd.row_reals * eb) {
     esk_fail(EINSPLINE_EINVAL, "table_load: %s: inconsistent header", path);
     return nullptr;

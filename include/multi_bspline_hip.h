@@ -266,6 +266,20 @@ int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
 enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
 int einspline_set_point_order(multi_UBspline_3d_hip *spline, int order);
 
+/* Which kernel evaluates vgh on a BF16 or FP16 table: the vector kernel
+ * (default; the tile applies) or the matrix-core kernel, which multiplies
+ * each point's weight matrix (one row per output: v, gradient, Hessian; 64
+ * stencil columns) with its 64 stencil rows as MFMA tiles, FP32
+ * accumulation (strided einspline_eval_vgh_batch of real tables only; other
+ * modes and entry points use the vector kernel; the store policy's
+ * streaming/ordinary choice applies).  The autotuner tries both.  Other
+ * storage types ignore the setting.  einspline_get_kernel returns the
+ * setting or a negative code.  No counterpart in the reference (its
+ * kernels are FP32 vector code). */
+enum { EINSPLINE_KERNEL_VECTOR = 0, EINSPLINE_KERNEL_MATRIX = 1 };
+int einspline_set_kernel(multi_UBspline_3d_hip *spline, int kernel);
+int einspline_get_kernel(const multi_UBspline_3d_hip *spline);
+
 /* The table's current tile, store policy and point order (defaults filled in). */
 int einspline_get_tuning(const multi_UBspline_3d_hip *spline, int *lane_bytes, int *block_threads,
                          int *store_policy, int *point_order);
