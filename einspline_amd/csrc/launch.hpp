@@ -22,10 +22,12 @@ struct Launch {
 // FP16 storage, else BF16.
 hipError_t launch_mfma16(const Launch &L, const Params &P, bool f16, hipStream_t s);
 
-// The matrix-core kernel serves strided vgh of real 16-bit tables;
-// everything else runs on the vector kernel.
-inline bool use_mfma16(const Launch &L, const Params &P) {
-  (void)P;
+// FP32 tables on v_mfma_f32_16x16x4_f32.
+hipError_t launch_mfma32(const Launch &L, const Params &P, hipStream_t s);
+
+// The matrix-core kernels serve strided vgh of real BF16, FP16 and FP32
+// tables; everything else runs on the vector kernel.
+inline bool use_mfma(const Launch &L) {
   return L.kernel == 1 && L.mode == MODE_VGH && L.api == API_STRIDED && !L.cplx;
 }
 

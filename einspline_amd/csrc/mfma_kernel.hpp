@@ -84,6 +84,69 @@ template <> struct MfmaRows<true> {
 // 128-orbital slab in output layout (the full Hessian, the larger case).
 constexpr int kMfmaStage = 128 * (1 + 3 + 9);
 
+// The wavefront's point (blocks of four wavefronts, XCD runs and the
+// traversal order as in eval3d): its cell and basis values (value, first and
+// second derivative weights per axis).  False past the last point.
+__device__ __forceinline__ bool mfma_point(const Params &P, int &p, int &ix, int &iy, int &iz,
+                                           float (&wx)[3][4], float (&wy)[3][4], float (&wz)[3][4]) {
+  unsigned b = blockIdx.x;
+  if (P.xcd_runs) {
+    const unsigned nb = gridDim.x, qq = nb >> 3, rr = nb & 7u, x = b & 7u, l = b >> 3;
+    b = (x < rr ? x * (qq + 1) : rr * (qq + 1) + (x - rr) * qq) + l;
+  }
+  p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (p >= P.npoints) return false;
+  if (P.perm) p = uniform(P.perm[p]);
+  const double *pp = (const double *)P.pos + 3l * p;
+  float tx, ty, tz;
+  locate<float>(pp[0], P.x0, P.xi, P.nx4, ix, tx);
+  locate<float>(pp[1], P.y0, P.yi, P.ny4, iy, ty);
+  locate<float>(pp[2], P.z0, P.zi, P.nz4, iz, tz);
+  ix = uniform(ix); iy = uniform(iy); iz = uniform(iz);
+  tx = uniform(tx); ty = uniform(ty); tz = uniform(tz);
+  basis0(tx, wx[0]); basis1(tx, wx[1]); basis2(tx, wx[2]);
+  basis0(ty, wy[0]); basis1(ty, wy[1]); basis2(ty, wy[2]);
+  basis0(tz, wz[0]); basis1(tz, wz[1]); basis2(tz, wz[2]);
+  return true;
+}
+
+// Output scale of accumulator entries e = 0..3 of lane group g (output rows
+// 4g + e): derivatives in physical units, times `oscale`.
+template <class Rows>
+__device__ __forceinline__ void mfma_scales(const Params &P, int g, float oscale, float (&sc)[4]) {
+  const float dxi = (float)P.xi, dyi = (float)P.yi, dzi = (float)P.zi;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int qe = 4 * g + e;
+    const int ex = (Rows::OX >> (2 * qe)) & 3, ey = (Rows::OY >> (2 * qe)) & 3, ez = (Rows::OZ >> (2 * qe)) & 3;
+    float f = oscale;
+    f = ex >= 1 ? f * dxi : f; f = ex == 2 ? f * dxi : f;
+    f = ey >= 1 ? f * dyi : f; f = ey == 2 ? f * dyi : f;
+    f = ez >= 1 ? f * dzi : f; f = ez == 2 ? f * dzi : f;
+    sc[e] = f;
+  }
+}
+
+// Choices are made between values, never between array elements: a
+// conditional over two lvalues selects an address, and the array then lives
+// in scratch memory.
+__device__ __forceinline__ float by_order(int o, float w0, float w1, float w2) {
+  return o == 0 ? w0 : o == 1 ? w1 : w2;
+}
+
+// Copy n floats of a staged block to dst: 16-byte pieces when dst is
+// aligned, streaming stores if nt.
+__device__ __forceinline__ void mfma_copy_out(int lane, const float *src, float *dst, int n, bool nt) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  int done = 0;
+  if (((uintptr_t)dst & 15) == 0) {
+    const int n4 = n >> 2;
+    for (int i = lane; i < n4; i += 64) st_out(*(const v4 *)(src + 4 * i), (v4 *)(dst + 4 * i), nt);
+    done = 4 * n4;
+  }
+  for (int i = done + lane; i < n; i += 64) dst[i] = src[i];
+}
+
 // Register budget: three wavefronts per SIMD (at most 168 VGPRs; the
 // kernel needs about 150 without spills, while four would spill).
 template <class S, bool PACKED>
@@ -97,27 +160,9 @@ eval3d_mfma(const Params P) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, r = lane & 15;
   float *const stage = stage_all + (threadIdx.x >> 6) * kMfmaStage;
-  unsigned b = blockIdx.x;
-  if (P.xcd_runs) {
-    const unsigned nb = gridDim.x, qq = nb >> 3, rr = nb & 7u, x = b & 7u, l = b >> 3;
-    b = (x < rr ? x * (qq + 1) : rr * (qq + 1) + (x - rr) * qq) + l;
-  }
-  int p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-  if (p >= P.npoints) return;
-  if (P.perm) p = uniform(P.perm[p]);
-
-  const double *pp = (const double *)P.pos + 3l * p;
-  int ix, iy, iz;
-  float tx, ty, tz;
-  locate<float>(pp[0], P.x0, P.xi, P.nx4, ix, tx);
-  locate<float>(pp[1], P.y0, P.yi, P.ny4, iy, ty);
-  locate<float>(pp[2], P.z0, P.zi, P.nz4, iz, tz);
-  ix = uniform(ix); iy = uniform(iy); iz = uniform(iz);
-  tx = uniform(tx); ty = uniform(ty); tz = uniform(tz);
+  int p, ix, iy, iz;
   float wx[3][4], wy[3][4], wz[3][4];
-  basis0(tx, wx[0]); basis1(tx, wx[1]); basis2(tx, wx[2]);
-  basis0(ty, wy[0]); basis1(ty, wy[1]); basis2(ty, wy[2]);
-  basis0(tz, wz[0]); basis1(tz, wz[1]); basis2(tz, wz[2]);
+  if (!mfma_point(P, p, ix, iy, iz, wx, wy, wz)) return;
 
   // FP16 holds magnitudes up to 65504 only: points outside the grid box
   // extrapolate with weights that grow like t^3 per axis, so products bounded
@@ -151,10 +196,6 @@ eval3d_mfma(const Params P) {
   // with a lane-dependent value.
   const int q = r;
   const int ox = (Rows::OX >> (2 * q)) & 3, oy = (Rows::OY >> (2 * q)) & 3, oz = (Rows::OZ >> (2 * q)) & 3;
-  // (choices are made between values, never between array elements: a
-  // conditional over two lvalues selects an address, and the array then
-  // lives in scratch memory)
-  auto by_order = [](int o, float w0, float w1, float w2) { return o == 0 ? w0 : o == 1 ? w1 : w2; };
   const bool gx1 = (g >> 1) != 0, gy1 = (g & 1) != 0;
 #define ESK_PICK(o, w, u) by_order(o, w[0][u], w[1][u], w[2][u])
   const float x0 = ESK_PICK(ox, wx, 0), x1 = ESK_PICK(ox, wx, 1), x2 = ESK_PICK(ox, wx, 2), x3 = ESK_PICK(ox, wx, 3);
@@ -180,20 +221,9 @@ eval3d_mfma(const Params P) {
   }
 
   // Accumulator entry e of lane (g, r) is output row 4g + e of orbital
-  // 8r + t (tile t) of the slab; its output scale (derivatives in physical
-  // units):
-  const float dxi = (float)P.xi, dyi = (float)P.yi, dzi = (float)P.zi;
+  // 8r + t (tile t) of the slab.
   float sc[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int qe = 4 * g + e;
-    const int ex = (Rows::OX >> (2 * qe)) & 3, ey = (Rows::OY >> (2 * qe)) & 3, ez = (Rows::OZ >> (2 * qe)) & 3;
-    float f = oscale;
-    f = ex >= 1 ? f * dxi : f; f = ex == 2 ? f * dxi : f;
-    f = ey >= 1 ? f * dyi : f; f = ey == 2 ? f * dyi : f;
-    f = ez >= 1 ? f * dzi : f; f = ez == 2 ? f * dzi : f;
-    sc[e] = f;
-  }
+  mfma_scales<Rows>(P, g, oscale, sc);
   // The staging area holds the slab in output layout: v block (128 floats),
   // gradient block (384), Hessian block (128 HS).  Lanes g = 0 hold v and
   // the gradient of their eight orbitals (8 and 24 consecutive floats),
@@ -247,17 +277,6 @@ eval3d_mfma(const Params P) {
       acc[t] = M::mma(c == 0 ? alo[0] : alo[1], bf, acc[t]);
     }
   };
-  auto copy_out = [&](const float *src, float *dst, int n) {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    int done = 0;
-    if (((uintptr_t)dst & 15) == 0) {
-      const int n4 = n >> 2;
-      for (int i = lane; i < n4; i += 64) st_out(*(const v4 *)(src + 4 * i), (v4 *)(dst + 4 * i), nt);
-      done = 4 * n4;
-    }
-    for (int i = done + lane; i < n; i += 64) dst[i] = src[i];
-  };
-
   // Software pipeline over half-slabs: the rows of the next half are in
   // flight while the current one is multiplied and the slab is stored.
   uint4 R0[8], R1[8];
@@ -294,9 +313,121 @@ eval3d_mfma(const Params P) {
     }
     wave_sync();
     const int m = min(128, P.nreal - s0);             // valid orbitals of the slab
-    copy_out(stage, ov + s0, m);
-    copy_out(stage + 128, og + 3 * s0, 3 * m);
-    copy_out(stage + 512, oh + Rows::HS * s0, Rows::HS * m);
+    mfma_copy_out(lane, stage, ov + s0, m, nt);
+    mfma_copy_out(lane, stage + 128, og + 3 * s0, 3 * m, nt);
+    mfma_copy_out(lane, stage + 512, oh + Rows::HS * s0, Rows::HS * m, nt);
+    wave_sync();                                      // reads done before the next slab's writes
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 tables: the same product on v_mfma_f32_16x16x4_f32, FP32 throughout.
+//
+// Operand layout (lane l, g = l / 16, r = l % 16): A holds W[r][g], B holds
+// P[g][column r], D holds D[4g .. 4g+3][column r].  The 64 stencil rows are
+// taken in 16 chunks of four, chunk kc = 4 i + j holding rows (i, j, k = 0..3),
+// so lane group g carries z row k = g.  Lane (g, r) loads 16 bytes (four
+// orbitals, 4r .. 4r+3 of a 64-orbital slab) of its row of each chunk, and
+// tile t (0..3) takes element t: column r is orbital 4r + t.  No data moves
+// between or inside lanes before the products.
+constexpr int kMfma32Stage = 64 * (1 + 3 + 9);
+
+// Four wavefronts per SIMD: about 126 VGPRs, no spills (five would spill).
+template <bool PACKED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+eval3d_mfma32(const Params P) {
+  using Rows = MfmaRows<PACKED>;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __shared__ __align__(16) float stage_all[4 * kMfma32Stage];
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, r = lane & 15;
+  float *const stage = stage_all + (threadIdx.x >> 6) * kMfma32Stage;
+  int p, ix, iy, iz;
+  float wx[3][4], wy[3][4], wz[3][4];
+  if (!mfma_point(P, p, ix, iy, iz, wx, wy, wz)) return;
+
+  // A operand of chunk kc: W[q = r][(i, j, g)] with i = kc / 4, j = kc % 4
+  const int q = r;
+  const int ox = (Rows::OX >> (2 * q)) & 3, oy = (Rows::OY >> (2 * q)) & 3, oz = (Rows::OZ >> (2 * q)) & 3;
+#define ESK_PICK(o, w, u) by_order(o, w[0][u], w[1][u], w[2][u])
+  const float x0 = ESK_PICK(ox, wx, 0), x1 = ESK_PICK(ox, wx, 1), x2 = ESK_PICK(ox, wx, 2), x3 = ESK_PICK(ox, wx, 3);
+  const float y0 = ESK_PICK(oy, wy, 0), y1 = ESK_PICK(oy, wy, 1), y2 = ESK_PICK(oy, wy, 2), y3 = ESK_PICK(oy, wy, 3);
+  const float z0 = ESK_PICK(oz, wz, 0), z1 = ESK_PICK(oz, wz, 1), z2 = ESK_PICK(oz, wz, 2), z3 = ESK_PICK(oz, wz, 3);
+#undef ESK_PICK
+  const float zg = q < Rows::NQ ? (g == 0 ? z0 : g == 1 ? z1 : g == 2 ? z2 : z3) : 0.0f;
+  float a[16];
+#pragma unroll
+  for (int kc = 0; kc < 16; ++kc) {
+    const int i = kc >> 2, j = kc & 3;
+    const float xi = i == 0 ? x0 : i == 1 ? x1 : i == 2 ? x2 : x3;
+    const float yj = j == 0 ? y0 : j == 1 ? y1 : j == 2 ? y2 : y3;
+    a[kc] = xi * yj * zg;
+  }
+  float sc[4];
+  mfma_scales<Rows>(P, g, 1.0f, sc);
+  // staging area in output layout for a 64-orbital slab: v (64 floats),
+  // gradient (192), Hessian (64 HS)
+  float *const vg_base = stage + 4 * r;                          // v of orbital 4r
+  float *const gr_base = stage + 64 + 12 * r;                    // gradient of orbital 4r
+  float *const h_base = stage + 256 + Rows::HS * 4 * r + 4 * (g - 1);
+
+  const float *cell = (const float *)P.coefs + (long)ix * P.xs + (long)iy * P.ys + (long)iz * P.zs;
+  const int row_reals = (int)P.zs;
+  const int end = min(row_reals, P.nreal);
+  const bool nt = P.store == STORE_STAGED_NT;
+  float *ov = (float *)P.out_v + (long)p * P.ld_v;
+  float *og = (float *)P.out_g + (long)p * P.ld_g;
+  float *oh = (float *)P.out_h + (long)p * P.ld_h;
+  const float *lane_row = cell + (long)g * P.zs;              // z row k = g
+  // rows of chunks 8h .. 8h+7 at orbitals s0 + 4r (rows are padded to
+  // whole 32-byte pieces, so a lane whose first orbital is inside the row
+  // reads inside it; other lanes read zeros)
+  auto load_half = [&](f4 (&R)[8], int s0, int h) {
+    const int o0 = s0 + 4 * r;
+    const bool have = o0 < row_reals;
+    const float *base = lane_row + o0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kc = 8 * h + u, i = kc >> 2, j = kc & 3;
+      R[u] = have ? *(const f4 *)(base + i * P.xs + j * P.ys) : f4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  auto mma_half = [&](const f4 (&R)[8], f4 (&acc)[4], int h) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(h == 0 ? a[u] : a[8 + u], R[u][t], acc[t], 0, 0, 0);
+  };
+
+  f4 R0[8], R1[8];
+  load_half(R0, 0, 0);
+  for (int s0 = 0; s0 < end; s0 += 64) {
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    load_half(R1, s0, 1);
+    mma_half(R0, acc, 0);
+    if (s0 + 64 < end) load_half(R0, s0 + 64, 0);
+    mma_half(R1, acc, 1);
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    if (g == 0) {
+      *(v4 *)vg_base = v4{acc[0][0], acc[1][0], acc[2][0], acc[3][0]} * sc[0];
+      *(v4 *)gr_base = v4{acc[0][1] * sc[1], acc[0][2] * sc[2], acc[0][3] * sc[3], acc[1][1] * sc[1]};
+      *(v4 *)(gr_base + 4) = v4{acc[1][2] * sc[2], acc[1][3] * sc[3], acc[2][1] * sc[1], acc[2][2] * sc[2]};
+      *(v4 *)(gr_base + 8) = v4{acc[2][3] * sc[3], acc[3][1] * sc[1], acc[3][2] * sc[2], acc[3][3] * sc[3]};
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (4 * g + e < Rows::NQ) h_base[Rows::HS * t + e] = acc[t][e] * sc[e];
+    }
+    wave_sync();
+    const int m = min(64, P.nreal - s0);              // valid orbitals of the slab
+    mfma_copy_out(lane, stage, ov + s0, m, nt);
+    mfma_copy_out(lane, stage + 64, og + 3 * s0, 3 * m, nt);
+    mfma_copy_out(lane, stage + 256, oh + Rows::HS * s0, Rows::HS * m, nt);
     wave_sync();                                      // reads done before the next slab's writes
   }
 }

@@ -557,9 +557,10 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
       }
     }
   }
-  // 16-bit tables, vgh: the matrix-core kernel in either order, streaming
-  // or ordinary stores (it has no tile)
-  if (narrow && mode == esk::MODE_VGH) {
+  // real BF16, FP16 and FP32 tables, vgh: the matrix-core kernel in either
+  // order, streaming or ordinary stores (it has no tile)
+  const bool cplx = t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX;
+  if (t->storage != EINSPLINE_STORE_F64 && !cplx && mode == esk::MODE_VGH) {
     for (int cand = 0; rc == EINSPLINE_OK && cand < 2 * norders; ++cand) {
       const int order = cand >> 1, nt = cand & 1 ? esk::STORE_STAGED : esk::STORE_STAGED_NT;
       ext->kernel = EINSPLINE_KERNEL_MATRIX;

@@ -266,16 +266,17 @@ int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
 enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
 int einspline_set_point_order(multi_UBspline_3d_hip *spline, int order);
 
-/* Which kernel evaluates vgh on a BF16 or FP16 table: the vector kernel
- * (default; the tile applies) or the matrix-core kernel, which multiplies
- * each point's weight matrix (one row per output: v, gradient, Hessian; 64
- * stencil columns) with its 64 stencil rows as MFMA tiles, FP32
+/* Which kernel evaluates vgh on a BF16, FP16 or FP32 table: the vector
+ * kernel (default; the tile applies) or the matrix-core kernel, which
+ * multiplies each point's weight matrix (one row per output: v, gradient,
+ * Hessian; 64 stencil columns) with its 64 stencil rows as MFMA tiles, FP32
  * accumulation (strided einspline_eval_vgh_batch of real tables only; other
  * modes and entry points use the vector kernel; the store policy's
- * streaming/ordinary choice applies).  The autotuner tries both.  Other
- * storage types ignore the setting.  einspline_get_kernel returns the
- * setting or a negative code.  No counterpart in the reference (its
- * kernels are FP32 vector code). */
+ * streaming/ordinary choice applies).  Its results differ from the vector
+ * kernel's in rounding only.  The autotuner tries both.  FP64 and complex
+ * tables ignore the setting.  einspline_get_kernel returns the setting or a
+ * negative code.  No counterpart in the reference (its kernels are vector
+ * code). */
 enum { EINSPLINE_KERNEL_VECTOR = 0, EINSPLINE_KERNEL_MATRIX = 1 };
 int einspline_set_kernel(multi_UBspline_3d_hip *spline, int kernel);
 int einspline_get_kernel(const multi_UBspline_3d_hip *spline);

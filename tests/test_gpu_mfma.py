@@ -1,14 +1,16 @@
-"""vgh of BF16 / FP16 tables on the matrix cores (einspline_set_kernel with
-EINSPLINE_KERNEL_MATRIX) against the oracle evaluated in FP64 on the table's
-own 16-bit coefficients, and against the vector kernel on the same table.
+"""vgh of BF16 / FP16 / FP32 tables on the matrix cores (einspline_set_kernel
+with EINSPLINE_KERNEL_MATRIX) against the oracle evaluated in FP64 on the
+table's own coefficients, and against the vector kernel on the same table.
 
-Error budget: the kernel splits every basis-weight product into a 16-bit
-high part and a 16-bit remainder (two products per tile), so each weight
-carries a relative error of at most 2^-18 (BF16) or 2^-24 (FP16) plus FP32
-accumulation; against the FP64 evaluation of the same 16-bit table the
+Error budget: the 16-bit kernel splits every basis-weight product into a
+16-bit high part and a 16-bit remainder (two products per tile), so each
+weight carries a relative error of at most 2^-18 (BF16) or 2^-24 (FP16) plus
+FP32 accumulation; against the FP64 evaluation of the same 16-bit table the
 normwise error is therefore bounded well under 1e-4 (BF16) and 1e-5 (FP16).
-The vector kernel computes the weights in FP32, so the two kernels agree to
-the same bars."""
+The FP32 kernel forms the weights and products in FP32 like the vector
+kernel (only the summation order differs): the FP32 bar of the other parity
+tests, 1e-5.  The vector kernel computes the weights in FP32, so the two
+kernels agree to the same bars."""
 import numpy as np
 import pytest
 
@@ -17,7 +19,8 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = {ea.STORE_BF16: 1e-4, ea.STORE_F16: 1e-5}
+TOL = {ea.STORE_BF16: 1e-4, ea.STORE_F16: 1e-5, ea.STORE_F32: 1e-5}
+STORAGES = [ea.STORE_BF16, ea.STORE_F16, ea.STORE_F32]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -37,10 +40,12 @@ def table_values(tab):
     """The device table's coefficients as FP64, shaped (Nx, Ny, Nz, N)."""
     t = tab.ptr.contents
     rows = t.Nx * t.Ny * t.Nz
-    raw = np.empty((rows, t.row_reals), np.uint16)
+    raw = np.empty((rows, t.row_reals), np.float32 if t.storage == ea.STORE_F32 else np.uint16)
     ea.check(ea.lib.einspline_hip_memcpy_dtoh(raw.ctypes.data, t.coefs, raw.nbytes), "dtoh")
     raw = raw[:, :t.num_splines]
-    if t.storage == ea.STORE_BF16:
+    if t.storage == ea.STORE_F32:
+        vals = raw.astype(np.float64)
+    elif t.storage == ea.STORE_BF16:
         vals = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
     else:
         vals = raw.view(np.float16).astype(np.float64)
@@ -72,7 +77,7 @@ def full_to_packed(H):
     return H[..., [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]
 
 
-@pytest.mark.parametrize("storage", [ea.STORE_BF16, ea.STORE_F16])
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("N", [5, 40, 128, 200, 512])
 @pytest.mark.parametrize("packed", [False, True])
 def test_matrix_kernel_matches_oracle_and_vector_kernel(storage, N, packed, rng):
@@ -94,7 +99,7 @@ def test_matrix_kernel_matches_oracle_and_vector_kernel(storage, N, packed, rng)
         np.testing.assert_array_equal(mH, np.swapaxes(mH, 2, 3))
 
 
-@pytest.mark.parametrize("storage", [ea.STORE_BF16, ea.STORE_F16])
+@pytest.mark.parametrize("storage", STORAGES)
 def test_matrix_kernel_locality_order_is_bitwise_identical(storage, rng):
     h, tab, o = build(storage, (16, 16, 16), 96, rng)
     pts = rng.random((5000, 3))
@@ -110,7 +115,7 @@ def test_matrix_kernel_locality_order_is_bitwise_identical(storage, rng):
     assert normwise(b[2][idx], rH) < TOL[storage]
 
 
-@pytest.mark.parametrize("storage", [ea.STORE_BF16, ea.STORE_F16])
+@pytest.mark.parametrize("storage", STORAGES)
 def test_matrix_kernel_nonperiodic_and_edge_points(storage, rng):
     bcs = [(ea.NATURAL, ea.FLAT), (ea.DERIV1, ea.DERIV2, 0.5, -1.0), (ea.PERIODIC, ea.PERIODIC)]
     start, end = (-1.0, 0.5, 2.0), (1.0, 3.0, 2.5)
@@ -154,20 +159,30 @@ def test_kernel_setting_api(rng):
         res.append(tab.download("vgl", 64, outs))
     for x, y in zip(*res):
         np.testing.assert_array_equal(x, y)
-    # FP32 tables accept and ignore the setting
-    h32 = ea.HostSpline("s", (6, 6, 6), (0, 0, 0), (1, 1, 1), 8)
-    h32.set_block(0, rng.uniform(-1, 1, (8, 6, 6, 6)))
-    t32 = ea.DeviceTable(h32)
-    a = eval_vgh(t32, pts, ea.KERNEL_VECTOR)
-    b = eval_vgh(t32, pts, ea.KERNEL_MATRIX)
-    for x, y in zip(a, b):
-        np.testing.assert_array_equal(x, y)
+    # FP64 and complex tables accept and ignore the setting
+    for kind in ("d", "c"):
+        hk = ea.HostSpline(kind, (6, 6, 6), (0, 0, 0), (1, 1, 1), 8)
+        data = rng.uniform(-1, 1, (8, 6, 6, 6))
+        hk.set_block(0, data + (1j * data[::-1] if kind == "c" else 0))
+        tk = ea.DeviceTable(hk)
+        P = len(pts)
+        posk = ea.DeviceBuffer.from_array(pts)
+        res = []
+        for k in (ea.KERNEL_VECTOR, ea.KERNEL_MATRIX):
+            tk.set_kernel(k)
+            outs = tk.alloc_outputs("vgh", P)
+            tk.eval_batch("vgh", posk, P, outs)
+            ea.synchronize()
+            res.append(tk.download("vgh", P, outs))
+        for x, y in zip(*res):
+            np.testing.assert_array_equal(x, y)
     assert ea.lib.einspline_set_kernel(None, 0) == ea.E_INVAL
     assert ea.lib.einspline_get_kernel(None) == ea.E_INVAL
 
 
-def test_autotune_may_pick_the_matrix_kernel_and_keeps_results(rng):
-    h, tab, o = build(ea.STORE_BF16, (24, 24, 24), 256, rng)
+@pytest.mark.parametrize("storage", [ea.STORE_BF16, ea.STORE_F32])
+def test_autotune_may_pick_the_matrix_kernel_and_keeps_results(storage, rng):
+    h, tab, o = build(storage, (24, 24, 24), 256, rng)
     tab.autotune("vgh", 8192)
     k = tab.kernel
     assert k in (ea.KERNEL_VECTOR, ea.KERNEL_MATRIX)
@@ -175,4 +190,4 @@ def test_autotune_may_pick_the_matrix_kernel_and_keeps_results(rng):
     rv, rg, rH = o.eval_batch("vgh", pts)
     tab_k = tab.kernel
     v, g, H = eval_vgh(tab, pts, tab_k)
-    assert normwise(v, rv) < TOL[ea.STORE_BF16] and normwise(H, rH) < TOL[ea.STORE_BF16]
+    assert normwise(v, rv) < TOL[storage] and normwise(H, rH) < TOL[storage]

@@ -132,7 +132,11 @@ def test_autotune_reports_an_order_and_keeps_results(rng):
     tab.autotune("vgh", 4096)
     lb, blk, pol, order = tab.tuning()
     assert order in (ea.ORDER_GIVEN, ea.ORDER_LOCALITY)
+    # the matrix-core kernel (a candidate for vgh) sums in another order
     for a, b in zip(gpu_eval(tab, "vgh", pts), before):
-        np.testing.assert_array_equal(a, b)
+        if tab.kernel == ea.KERNEL_VECTOR:
+            np.testing.assert_array_equal(a, b)
+        else:
+            assert normwise(a, b) < TOL["s"]
     with pytest.raises(ea.EinsplineError):
         tab.set_point_order(7)

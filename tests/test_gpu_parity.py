@@ -262,8 +262,13 @@ def test_autotune_picks_a_valid_tile_and_keeps_results(rng):
     lb, blk, nt, ms = tab.autotune("vgh", 4096)
     assert lb in (8, 16, 32) and blk in (64, 128, 256) and nt in (0, 1, 2) and ms > 0
     after = gpu_eval(tab, "vgh", pts)
+    # tiles and orders never change results; the matrix-core kernel (which
+    # the tuner may pick for vgh) sums in another order, within the FP32 bar
     for a, b in zip(after, before):
-        np.testing.assert_array_equal(a, b)
+        if tab.kernel == ea.KERNEL_VECTOR:
+            np.testing.assert_array_equal(a, b)
+        else:
+            assert normwise(a, b) < TOL["s"]
     # second call on the same shape hits the cache
     tab2 = ea.DeviceTable(h)
     lb2, blk2, nt2, ms2 = tab2.autotune("vgh", 4096)
