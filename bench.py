@@ -43,6 +43,9 @@ METRIC = "orbital-vgh evals/sec + %HBM roofline, FP32 vs BF16 table; serving p99
 ORDER_NAMES = ("given", "locality")
 
 
+KERNELS = ("vector", "matrix")
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -166,6 +169,7 @@ def headline(args):
         tab.set_tile(args.lane_bytes, args.block)
         tab.set_store_policy(args.store_policy)
         tab.set_point_order(max(args.point_order, 0))
+        tab.set_kernel(KERNELS.index(args.kernel) if args.kernel else ea.KERNEL_VECTOR)
     else:
         tab.autotune("vgh", P)
         if args.point_order >= 0:
@@ -184,8 +188,9 @@ def headline(args):
     # measured HBM traffic of the same call (rocprofv3 PMC, profiles/); used
     # only when it was taken at this tile and point order
     traffic = load_traffic(f"eval3d_f32_vgh_48c_n256_p32768.{ORDER_NAMES[tile[3]]}.traffic.json")
-    if traffic and (traffic.get("tile", {}).get("lane_bytes"), traffic.get("tile", {}).get("block"),
-                    traffic.get("tile", {}).get("store_policy")) != tuple(tile[:3]):
+    if traffic and ((traffic.get("tile", {}).get("lane_bytes"), traffic.get("tile", {}).get("block"),
+                     traffic.get("tile", {}).get("store_policy")) != tuple(tile[:3])
+                    or tab.kernel != ea.KERNEL_VECTOR):
         traffic = None
     flops = evals * FLOPS_PER_EVAL["vgh"]
     res = {
@@ -208,7 +213,7 @@ def headline(args):
                    "table": "f32", "hessian_layout": "packed6" if packed else "full9",
                    "table_bytes": int(tab.bytes), "tile_lane_bytes": tile[0],
                    "tile_block": tile[1], "store_policy": tile[2],
-                   "point_order": ORDER_NAMES[tile[3]]},
+                   "point_order": ORDER_NAMES[tile[3]], "kernel": KERNELS[tab.kernel]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
@@ -578,6 +583,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--store-policy", type=int, default=0,
                     help="with --lane-bytes/--block: 0 staged+streaming, 1 staged, 2 direct")
+    ap.add_argument("--kernel", choices=KERNELS, default=None,
+                    help="with --lane-bytes/--block: the headline kernel (default vector)")
     ap.add_argument("--point-order", type=int, default=-1,
                     help="0 given, 1 locality (default: the autotuner's choice)")
     args = ap.parse_args()
