@@ -286,6 +286,19 @@ __device__ __forceinline__ void store_partial(A *dst, const A *r, int m) {
     if (e < m) st1(dst + e, r[e]);
 }
 
+// Logical block of hardware block b of nb.  Workgroups are dispatched to the
+// 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its own L2.
+// With `runs`, XCD x takes the x-th contiguous run of logical blocks, so
+// consecutive slots of a sorted traversal share that XCD's L2.  (Dealing
+// chunks of 16-256 consecutive blocks round-robin instead, so that all XCDs
+// walk one region of the sorted order together, measured the same or slower
+// on MI355X.)
+__device__ __forceinline__ unsigned logical_block(unsigned b, unsigned nb, int runs) {
+  if (!runs) return b;
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
 // Make LDS writes of this wavefront visible to its other lanes.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -842,14 +855,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
 template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS>
 __global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) eval3d(const Params P) {
   // npoints * chunks < 2^31 is checked on the host, so 32-bit math suffices
-  unsigned b = blockIdx.x;
-  if (P.xcd_runs) {
-    // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on
-    // XCD b % 8).  Give XCD x the x-th contiguous run of logical blocks, so
-    // consecutive slots of a sorted traversal share that XCD's L2.
-    const unsigned nb = gridDim.x, q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
-    b = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
-  }
+  const unsigned b = logical_block(blockIdx.x, gridDim.x, P.xcd_runs);
   const unsigned gid = b * blockDim.x + threadIdx.x;
   int p = (int)(gid / (unsigned)P.chunks);
   if (UNIF) p = uniform(p);

@@ -89,11 +89,7 @@ constexpr int kMfmaStage = 128 * (1 + 3 + 9);
 // second derivative weights per axis).  False past the last point.
 __device__ __forceinline__ bool mfma_point(const Params &P, int &p, int &ix, int &iy, int &iz,
                                            float (&wx)[3][4], float (&wy)[3][4], float (&wz)[3][4]) {
-  unsigned b = blockIdx.x;
-  if (P.xcd_runs) {
-    const unsigned nb = gridDim.x, qq = nb >> 3, rr = nb & 7u, x = b & 7u, l = b >> 3;
-    b = (x < rr ? x * (qq + 1) : rr * (qq + 1) + (x - rr) * qq) + l;
-  }
+  const unsigned b = logical_block(blockIdx.x, gridDim.x, P.xcd_runs);
   p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (p >= P.npoints) return false;
   if (P.perm) p = uniform(P.perm[p]);
