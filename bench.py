@@ -492,6 +492,50 @@ def _time_calls(fn, steps, warmup):
     return e0.elapsed_ms(e1) / steps
 
 
+def table_build(args, res):
+    """GPU table build (einspline_build_table_hip) against the host solve
+    (set_multi_UBspline_3d_d_block) plus the upload, on the same samples:
+    periodic 64^3, 256 orbitals of plane-wave data, FP64 table (and the same
+    build stored as BF16).  The two tables are equal byte for byte
+    (tests/test_gpu_build.py)."""
+    M, N = 64, 256
+    rng = np.random.default_rng(77)
+    data = np.concatenate([plane_wave_block(rng, M, 32) for _ in range(N // 32)])
+    h = ea.HostSpline("d", (M, M, M), (0, 0, 0), (1, 1, 1), N)
+    t0 = time.perf_counter()
+    h.set_block(0, data)
+    host_solve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tab = ea.DeviceTable(h)
+    ea.synchronize()
+    upload = time.perf_counter() - t0
+    table_bytes = int(tab.bytes)
+    tab.destroy()
+    buf = ea.DeviceBuffer.from_array(np.ascontiguousarray(data, np.float64))
+    rows = {}
+    for name, st in (("f64", -1), ("bf16", ea.STORE_BF16)):
+        best = None
+        for _ in range(3):
+            ea.synchronize()
+            t0 = time.perf_counter()
+            p = ea.lib.einspline_build_table_hip(h.ptr, buf.ptr, st, None)
+            ea.synchronize()
+            dt = time.perf_counter() - t0
+            if not p:
+                raise ea.EinsplineError(ea.E_INVAL, "einspline_build_table_hip")
+            ea.lib.destroy_multi_UBspline_3d_hip(p)
+            best = dt if best is None else min(best, dt)
+        rows[f"gpu_build_{name}_s"] = best
+    buf.free()
+    rows.update({"host_solve_s": host_solve, "host_upload_s": upload, "table_bytes_f64": table_bytes,
+                 "speedup_vs_host_solve_plus_upload": (host_solve + upload) / rows["gpu_build_f64_s"],
+                 "workload": "periodic 64^3, 256 orbitals, samples already on the device; "
+                             "host: set_multi_UBspline_3d_d_block (16 threads) + upload"})
+    log(f"[bench] table build 64^3 x 256: host solve {host_solve:.2f} s + upload {upload:.2f} s, "
+        f"GPU {rows['gpu_build_f64_s'] * 1e3:.1f} ms (bf16 {rows['gpu_build_bf16_s'] * 1e3:.1f} ms)")
+    res.setdefault("extra", {})["table_build"] = rows
+
+
 def determinant(args, res):
     """The Slater-determinant kernels that consume the orbital rows, at QMC
     sizes (include/determinant_update_hip.h):
@@ -576,6 +620,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the complex / FP64 configurations")
     ap.add_argument("--no-extra", action="store_true", help="skip the packed-Hessian side measurement")
     ap.add_argument("--no-det", action="store_true", help="skip the determinant kernels")
+    ap.add_argument("--no-build", action="store_true", help="skip the GPU table build measurement")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--packed", action="store_true", help="packed 6-entry Hessian outputs")
     ap.add_argument("--lane-bytes", type=int, default=0)
@@ -602,6 +647,8 @@ def main():
         other_configs(args, res)
     if not args.no_det:
         determinant(args, res)
+    if not args.no_build:
+        table_build(args, res)
     if not args.no_serving:
         serving(args, res)
     ex = res.get("extra", {})
@@ -617,6 +664,8 @@ def main():
         summ["serving_p99_ms_at_50pct"] = r["load_50pct"]["p99_ms"]
         summ["serving_p99_ms_at_90pct"] = r["load_90pct"]["p99_ms"]
         summ["serving_capacity_evals_per_s_at_p99_1ms"] = r["capacity_evals_per_s"]
+    if "table_build" in ex:
+        summ["gpu_table_build_speedup_64c_n256"] = ex["table_build"]["speedup_vs_host_solve_plus_upload"]
     if "determinant" in ex:
         d = ex["determinant"]
         summ["det_update_f32_n256_w256_frac_hbm"] = d["update_f32_n256_w256"]["frac_hbm"]
