@@ -73,18 +73,12 @@ struct AxisDev {
   const double *arr;                   // inv, g, sub, w: M each
 };
 
-// One column per thread: elements e of the column are col[e*es], e < n.
-// Columns: line * line_stride + q for line < nlines, q < len.
-__global__ void __launch_bounds__(256) axis_sweep(double *__restrict__ G, AxisDev op, long nlines,
-                                                  long line_stride, long es, long len, int ncomp) {
+// The passes of one column (element accessor EL), as the host's axis solve.
+template <class ElFn>
+__device__ __forceinline__ void sweep_column(ElFn &EL, const AxisDev &op, int M, const double *inv,
+                                             const double *gg, const double *sub, const double *w, long q,
+                                             int ncomp) {
 #pragma clang fp contract(off)
-  const long id = (long)blockIdx.x * 256 + threadIdx.x;
-  if (id >= nlines * len) return;
-  const long line = id / len, q = id - line * len;
-  double *col = G + line * line_stride + q;
-  const int M = op.M;
-  const double *inv = op.arr, *gg = op.arr + M, *sub = op.arr + 2 * M, *w = op.arr + 3 * M;
-#define EL(e) col[(long)(e) * es]
   if (op.kind != 2) {
     const int m = M - 1;
     double prev = EL(1) * inv[0];
@@ -141,7 +135,46 @@ __global__ void __launch_bounds__(256) axis_sweep(double *__restrict__ G, AxisDe
   const double c1 = EL(1), c2 = EL(2), cM = EL(M), cMm = EL(M - 1);
   EL(0) = (d0 - kBeta * c1 - kAlpha * c2) / kAlpha;
   EL(M + 1) = (dl - kAlpha * cMm - kBeta * cM) / kAlpha;
-#undef EL
+}
+
+// One column per thread: elements e of the column are col[e*es], e < n (the
+// axis length of G).  Columns: line * line_stride + q for line < nlines,
+// q < len.  With LDS the thread copies its column into the block's dynamic
+// LDS (element e of thread t at e * blockDim + t), runs the passes there and
+// copies it back: one read and one write of G per axis instead of one per
+// pass.  The arithmetic is the same in both forms.
+template <bool LDS>
+__global__ void __launch_bounds__(256) axis_sweep(double *__restrict__ G, AxisDev op, long nlines,
+                                                  long line_stride, long es, long len, int ncomp, int n) {
+#pragma clang fp contract(off)
+  extern __shared__ double cl[];
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nlines * len) return;
+  const long line = id / len, q = id - line * len;
+  double *col = G + line * line_stride + q;
+  const int M = op.M;
+  const double *inv = op.arr, *gg = op.arr + M, *sub = op.arr + 2 * M, *w = op.arr + 3 * M;
+  const int T = blockDim.x, t = threadIdx.x;
+  auto EL = [&](long e) -> double & {
+    if constexpr (LDS) return cl[e * T + t];
+    else return col[e * es];
+  };
+  if constexpr (LDS) {
+    // eight loads in flight per thread
+    int e = 0;
+    for (; e + 8 <= n; e += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = col[(long)(e + u) * es];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cl[(long)(e + u) * T + t] = v[u];
+    }
+    for (; e < n; ++e) cl[(long)e * T + t] = col[(long)e * es];
+  }
+  sweep_column(EL, op, M, inv, gg, sub, w, q, ncomp);
+  if constexpr (LDS) {
+    for (int e = 0; e < n; ++e) col[(long)e * es] = cl[(long)e * T + t];
+  }
 }
 
 // G[e][q] (q < L) -> table row e, reals col0 + q, in the storage type
@@ -163,6 +196,8 @@ __global__ void __launch_bounds__(256) store_table(const double *__restrict__ G,
 }
 
 unsigned grid1(long n) { return (unsigned)std::min<long>((n + 255) / 256, 1l << 30); }
+
+constexpr size_t kSweepLds = 80 * 1024;   // LDS per sweep block: two blocks per CU
 
 }  // namespace
 
@@ -246,6 +281,27 @@ extern "C" multi_UBspline_3d_hip *einspline_build_table_hip(void *spline, const 
   if (rc == EINSPLINE_OK && ((e = hipMemsetAsync(table, 0, bytes, s)) != hipSuccess ||
                              (e = hipMemsetAsync(dovf, 0, sizeof(int), s)) != hipSuccess))
     rc = esk_hip_fail(e, "build_table: clear");
+  // One sweep along an axis of n coefficients: columns in LDS when the
+  // passes touch exactly the column's n elements and a block of at least 64
+  // columns fits the budget (two blocks per CU), else from G directly.
+  auto sweep = [&](const AxisDev &o, long nlines, long lstride, long es, long len, int n) {
+    const long cols = nlines * len;
+    const bool exact = (o.kind == 2 ? o.M + 2 : o.M + 3) == n;
+    int T = 256;
+    while (T > 64 && (size_t)n * T * sizeof(double) > kSweepLds) T /= 2;
+    const size_t lds = (size_t)n * T * sizeof(double);
+    if (exact && lds <= kSweepLds) {
+      hipLaunchKernelGGL(axis_sweep<true>, dim3((unsigned)std::min<long>((cols + T - 1) / T, 1l << 30)), dim3(T),
+                         lds, s, G, o, nlines, lstride, es, len, ncomp, n);
+    } else {
+      hipLaunchKernelGGL(axis_sweep<false>, dim3(grid1(cols)), dim3(256), 0, s, G, o, nlines, lstride, es, len,
+                         ncomp, n);
+    }
+  };
+  if (rc == EINSPLINE_OK &&
+      (e = hipFuncSetAttribute((const void *)axis_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kSweepLds)) != hipSuccess)
+    rc = esk_hip_fail(e, "build_table: LDS size");
   const Geom g0{ax[0].M, ax[1].M, ax[2].M, Nx, Ny, Nz, 0};
   const long Mvol = (long)g0.Mx * g0.My * g0.Mz;
   for (int o0 = 0; rc == EINSPLINE_OK && o0 < N; o0 += per) {
@@ -267,11 +323,9 @@ extern "C" multi_UBspline_3d_hip *einspline_build_table_hip(void *spline, const 
     }
     // x: one line of Ny*Nz*L columns; y: Nx lines of Nz*L; z: Nx*Ny lines of L
     const long plane = (long)Ny * Nz * L;
-    hipLaunchKernelGGL(axis_sweep, dim3(grid1(plane)), dim3(256), 0, s, G, op[0], 1l, 0l, plane, plane, ncomp);
-    hipLaunchKernelGGL(axis_sweep, dim3(grid1((long)Nx * Nz * L)), dim3(256), 0, s, G, op[1], (long)Nx, plane,
-                       (long)Nz * L, (long)Nz * L, ncomp);
-    hipLaunchKernelGGL(axis_sweep, dim3(grid1((long)Nx * Ny * L)), dim3(256), 0, s, G, op[2], (long)Nx * Ny,
-                       (long)Nz * L, L, L, ncomp);
+    sweep(op[0], 1l, 0l, plane, plane, Nx);
+    sweep(op[1], (long)Nx, plane, (long)Nz * L, (long)Nz * L, Ny);
+    sweep(op[2], (long)Nx * Ny, (long)Nz * L, L, L, Nz);
     hipLaunchKernelGGL(store_table, dim3(std::min<unsigned>(grid1((long)vol * L), 16384u)), dim3(256), 0, s, G,
                        table, (long)vol, L, (long)row, (long)o0 * ncomp, (int)st, dovf);
     if ((e = hipGetLastError()) != hipSuccess) rc = esk_hip_fail(e, "build_table: launch");
