@@ -635,7 +635,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the spline evaluator is a single-GPU kernel "
-              "(orbital sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding the orbitals across GPUs needs no reduction, but is not built)", file=sys.stderr)
         sys.exit(2)
     if ea.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
