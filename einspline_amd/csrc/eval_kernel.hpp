@@ -775,14 +775,20 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
     for (int e = 0; e < V; ++e) r[e] = acc[4][e];
     if (full) store_block<A, V>(oh + e0, r, vec, nt); else store_partial<A, V>(oh + e0, r, m);
   }
+  // Complex orbitals: the CPU layout interleaves re/im per entry.  Through
+  // pointer arrays the reference's CUDA kernels treat a complex table as 2N
+  // reals, so entries are [q][re,im][3] and [q][re,im][6]
+  // (src/multi_bspline_cuda_z_impl.h:205-228, src/multi_bspline_cuda_c_impl.h:389-412).
+  constexpr bool CPU_CPLX = CPLX && !PTRS;
   if constexpr (MODE >= MODE_VG) {
     // gradient: real orbital e -> [e][3]; complex orbital q -> [q][3][re,im]
+    // (the CPU layout), except through pointer arrays (CPU_CPLX false)
     A r[3 * V];
 #pragma unroll
     for (int e = 0; e < V; ++e)
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const int o = CPLX ? (e >> 1) * 6 + d * 2 + (e & 1) : e * 3 + d;
+        const int o = CPU_CPLX ? (e >> 1) * 6 + d * 2 + (e & 1) : e * 3 + d;
         r[o] = acc[1 + d][e];
       }
     if constexpr (STAGE) {
@@ -803,7 +809,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
       for (int e = 0; e < V; ++e)
 #pragma unroll
         for (int h = 0; h < 6; ++h) {
-          const int o = CPLX ? (e >> 1) * 12 + h * 2 + (e & 1) : e * 6 + h;
+          const int o = CPU_CPLX ? (e >> 1) * 12 + h * 2 + (e & 1) : e * 6 + h;
           r[o] = acc[pack6[h]][e];
         }
       if constexpr (STAGE) {
@@ -819,7 +825,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
       for (int e = 0; e < V; ++e)
 #pragma unroll
         for (int h = 0; h < 9; ++h) {
-          const int o = CPLX ? (e >> 1) * 18 + h * 2 + (e & 1) : e * 9 + h;
+          const int o = CPU_CPLX ? (e >> 1) * 18 + h * 2 + (e & 1) : e * 9 + h;
           r[o] = acc[full9[h]][e];
         }
       if constexpr (STAGE) {

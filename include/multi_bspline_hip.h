@@ -157,7 +157,11 @@ int eval_multi_multi_UBspline_1d_z_vgl_hip(multi_UBspline_1d_z_hip *spline, doub
  * vals_d[i], grads_d[i], hess_d[i] are device pointers (held in device
  * arrays) to point i's outputs.  Gradients are [n][3]; Hessians are packed
  * as 6 unique entries [xx,xy,xz,yy,yz,zz] per orbital, as the reference's
- * CUDA kernels write them; complex results interleave re/im per entry.
+ * CUDA kernels write them.  Complex tables are written as those kernels
+ * write them, as 2N reals: per orbital the gradient is [re x,y,z][im x,y,z]
+ * and the Hessian [re 6][im 6] (src/multi_bspline_cuda_z_impl.h:205-228,
+ * src/multi_bspline_cuda_c_impl.h:389-412); the complex_* pointer types only
+ * give the blocks' sizes.
  * These calls synchronise the device before returning, as the reference's
  * do. */
 int eval_multi_multi_UBspline_3d_s_hip(multi_UBspline_3d_s_hip *spline, float *pos_d,

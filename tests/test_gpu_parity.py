@@ -148,7 +148,10 @@ def test_packed_hessian_equals_full(rng):
 def test_cuda_style_pointer_arrays(kind, rng):
     """eval_multi_multi_UBspline_3d_X{,_vgh}_hip: device arrays of per-point
     pointers, packed 6-entry Hessian, float positions for s/c
-    (src/multi_bspline_eval_cuda.h:62-137)."""
+    (src/multi_bspline_eval_cuda.h:62-137).  Complex tables are written as
+    the reference's CUDA kernels write them, as 2N reals: value re, im;
+    gradient [re: x y z][im: x y z]; Hessian [re: 6][im: 6] per orbital
+    (src/multi_bspline_cuda_z_impl.h:205-228)."""
     M, N, P = (7, 7, 7), 24, 50
     h, o = make_pair(kind, M, N, rng)
     tab = ea.DeviceTable(h)
@@ -170,7 +173,11 @@ def test_cuda_style_pointer_arrays(kind, rng):
     rv, rg, rh = o.eval_batch("vgh", pts_used)
     if kind in "cz":
         cdt = np.complex64 if kind == "c" else np.complex128
-        v, g, hh = v.view(cdt), g.view(cdt), hh.view(cdt)
+        v = v.view(cdt)
+        g2 = g.reshape(P, N, 2, 3)
+        g = g2[:, :, 0, :] + 1j * g2[:, :, 1, :]
+        h2 = hh.reshape(P, N, 2, 6)
+        hh = h2[:, :, 0, :] + 1j * h2[:, :, 1, :]
     assert normwise(v.reshape(P, N), rv) < TOL[kind]
     assert normwise(g.reshape(P, N, 3), rg) < TOL[kind]
     iu = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
