@@ -83,21 +83,22 @@ __global__ void __launch_bounds__(kThreads) bin_points(const PosT *__restrict__ 
   rank[i] = atomicAdd(&counts[b], 1);
 }
 
-// Exclusive prefix sum of counts[0, nbins) into offs[] (LDS), by every block:
-// rounds of kThreads consecutive bins (one per thread, no LDS bank
+// Exclusive prefix sum of counts[0, nbins) into offs[] (LDS), by every block
+// of T threads: rounds of T consecutive bins (one per thread, no LDS bank
 // conflicts), each scanned across the wavefront with shuffles and across the
-// block's four wavefronts through LDS, with a running carry between rounds.
+// block's wavefronts through LDS, with a running carry between rounds.
+template <int T>
 __device__ __forceinline__ void block_offsets(const int *__restrict__ counts, int nbins, int *offs,
                                               int *wave_sum) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < kBins / kThreads; ++k) {
-    const int b = k * kThreads + t;
+  for (int k = 0; k < (kBins + T - 1) / T; ++k) {
+    const int b = k * T + t;
     if (b < nbins) offs[b] = counts[b];
   }
   __syncthreads();
   int carry = 0;
-  for (int k = 0; k < nbins; k += kThreads) {
+  for (int k = 0; k < nbins; k += T) {
     const int b = k + t;
     const int v = b < nbins ? offs[b] : 0;
     const int incl = wave_scan(v);
@@ -105,7 +106,7 @@ __device__ __forceinline__ void block_offsets(const int *__restrict__ counts, in
     __syncthreads();
     int before = 0, total = 0;
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) {
+    for (int w = 0; w < T / 64; ++w) {
       const int ws = wave_sum[w];
       if (w < (t >> 6)) before += ws;
       total += ws;
@@ -116,20 +117,23 @@ __device__ __forceinline__ void block_offsets(const int *__restrict__ counts, in
   }
 }
 
-__global__ void __launch_bounds__(kThreads) scan_scatter(const int *__restrict__ counts,
-                                                         int *__restrict__ next_counts, int nbins,
-                                                         const int *__restrict__ bin,
-                                                         const int *__restrict__ rank, int n,
-                                                         int *__restrict__ perm) {
+// Blocks of 1024 threads: the prefix over at most 2048 bins takes two rounds,
+// and a quarter as many blocks repeat it.
+constexpr int kScanThreads = 1024;
+
+__global__ void __launch_bounds__(kScanThreads) scan_scatter(const int *__restrict__ counts,
+                                                             int *__restrict__ next_counts, int nbins,
+                                                             const int *__restrict__ bin,
+                                                             const int *__restrict__ rank, int n,
+                                                             int *__restrict__ perm) {
   __shared__ int offs[kBins];
-  __shared__ int wave_sum[kThreads / 64];
+  __shared__ int wave_sum[kScanThreads / 64];
   if (blockIdx.x == 0) {
     // the next call counts into the other buffer; clear it now
-#pragma unroll
-    for (int k = 0; k < kBins / kThreads; ++k) next_counts[k * kThreads + threadIdx.x] = 0;
+    for (int k = threadIdx.x; k < kBins; k += kScanThreads) next_counts[k] = 0;
   }
-  block_offsets(counts, nbins, offs, wave_sum);
-  const int i = blockIdx.x * kThreads + threadIdx.x;
+  block_offsets<kScanThreads>(counts, nbins, offs, wave_sum);
+  const int i = blockIdx.x * kScanThreads + threadIdx.x;
   if (i < n) perm[offs[bin[i]] + rank[i]] = i;
 }
 
@@ -218,8 +222,8 @@ int esk_order_begin(EskOrderWs **slot, const esk::Params &P, const void *pos, bo
   else
     hipLaunchKernelGGL(bin_points<double>, dim3(blocks), dim3(kThreads), 0, s, (const double *)pos, num, P,
                        cell_shift, bin_shift, counts, bin, rank);
-  hipLaunchKernelGGL(scan_scatter, dim3(blocks), dim3(kThreads), 0, s, counts, next_counts, nbins, bin, rank,
-                     num, pm);
+  hipLaunchKernelGGL(scan_scatter, dim3((unsigned)((num + kScanThreads - 1) / kScanThreads)), dim3(kScanThreads), 0,
+                     s, counts, next_counts, nbins, bin, rank, num, pm);
   if ((e = hipGetLastError()) != hipSuccess) {
     w->cap = 0;                 // the counts may be dirty: reallocate and clear on the next call
     return bail(esk_hip_fail(e, "locality order launch"));
