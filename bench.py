@@ -186,12 +186,15 @@ def headline(args):
     achieved = evals * bpe / (kern * 1e-3) / 1e9
     log(f"[bench] FP32 vgh: {wall:.4f} ms/step wall, {kern:.4f} ms/launch (events)")
     # measured HBM traffic of the same call (rocprofv3 PMC, profiles/); used
-    # only when it was taken at this tile and point order
-    traffic = load_traffic(f"eval3d_f32_vgh_48c_n256_p32768.{ORDER_NAMES[tile[3]]}.traffic.json")
-    if traffic and ((traffic.get("tile", {}).get("lane_bytes"), traffic.get("tile", {}).get("block"),
-                     traffic.get("tile", {}).get("store_policy")) != tuple(tile[:3])
-                    or tab.kernel != ea.KERNEL_VECTOR):
-        traffic = None
+    # only when it was taken at this tile, store policy and point order
+    traffic = None
+    if tab.kernel == ea.KERNEL_VECTOR:
+        for suffix in ("", ".nt"):
+            t = load_traffic(f"eval3d_f32_vgh_48c_n256_p32768.{ORDER_NAMES[tile[3]]}{suffix}.traffic.json")
+            if t and (t.get("tile", {}).get("lane_bytes"), t.get("tile", {}).get("block"),
+                      t.get("tile", {}).get("store_policy")) == tuple(tile[:3]):
+                traffic = t
+                break
     flops = evals * FLOPS_PER_EVAL["vgh"]
     res = {
         "metric": METRIC,
@@ -266,6 +269,27 @@ def headline(args):
     return res
 
 
+def clustered(tab, P, args):
+    """The BF16 table on P points piled into [0, 0.05)^3 (about 5 cells per
+    axis): the configuration tuned on uniform points against the one that
+    einspline_autotune_points picks on a sample of these points."""
+    rng = np.random.default_rng(11)
+    pts = rng.random((P, 3)) * 0.05
+    pos = ea.DeviceBuffer.from_array(pts)
+    outs = tab.alloc_outputs("vgh", P)
+    steps = max(5, args.steps // 2)
+    uniform_cfg = (tab.tuning(), tab.kernel)
+    _, ms_uniform = time_batches(tab, "vgh", pos, P, outs, steps, 3)
+    tab.autotune_points("vgh", pts[:16384])
+    _, ms_own = time_batches(tab, "vgh", pos, P, outs, steps, 3)
+    own_cfg = (tab.tuning(), tab.kernel)
+    log(f"[bench] clustered BF16: uniform-tuned {ms_uniform:.3f} ms, tuned on the points {ms_own:.3f} ms")
+    return {"points": "uniform in [0, 0.05)^3 (about 5 cells per axis)",
+            "uniform_tuned_ms": ms_uniform, "uniform_tuned": {"tuning": list(uniform_cfg[0]), "kernel": uniform_cfg[1]},
+            "points_tuned_ms": ms_own, "points_tuned": {"tuning": list(own_cfg[0]), "kernel": own_cfg[1]},
+            "speedup": ms_uniform / ms_own}
+
+
 def precision(args, res):
     """96^3 periodic, 512 orbitals built in FP64, stored FP32/BF16/FP16;
     vgh at 131072 points; errors against the FP64 evaluation."""
@@ -321,6 +345,8 @@ def precision(args, res):
                       "kernel": "matrix" if tab.kernel == ea.KERNEL_MATRIX else "vector",
                       "error_vs_f64": errs}
         log(f"[bench] precision {name}: {kern:.3f} ms/launch, v max rel {errs['v']['max_rel']:.2e}")
+        if name == "bf16":
+            rows["clustered_points"] = clustered(tab, P, args)
         tab.destroy()
     rows["bf16_speedup_vs_f32"] = rows["f32"]["kernel_ms"] / rows["bf16"]["kernel_ms"]
     rows["f16_speedup_vs_f32"] = rows["f32"]["kernel_ms"] / rows["f16"]["kernel_ms"]

@@ -213,6 +213,8 @@ _proto("einspline_hip_invalidate", None, _vp)
 _proto("einspline_set_tile", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i)
 _proto("einspline_autotune", _i, C.POINTER(multi_UBspline_3d_hip), _i, _i, C.POINTER(_i),
        C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_float))
+_proto("einspline_autotune_points", _i, C.POINTER(multi_UBspline_3d_hip), _i, C.POINTER(C.c_double), _i,
+       C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_float))
 _proto("einspline_set_store_policy", _i, C.POINTER(multi_UBspline_3d_hip), _i)
 _proto("einspline_set_point_order", _i, C.POINTER(multi_UBspline_3d_hip), _i)
 _proto("einspline_set_kernel", _i, C.POINTER(multi_UBspline_3d_hip), _i)
@@ -640,6 +642,18 @@ class DeviceTable:
         lb, blk, nt, ms = _i(), _i(), _i(), C.c_float()
         check(lib.einspline_autotune(self.ptr, MODES.index(mode), int(sample_points), C.byref(lb),
                                      C.byref(blk), C.byref(nt), C.byref(ms)), "einspline_autotune")
+        return lb.value, blk.value, nt.value, ms.value
+
+    def autotune_points(self, mode, points):
+        """Like autotune, but timed on the caller's own host positions
+        (an (n, 3) array, a sample of the points to be evaluated), so the
+        point order suits their distribution.  Not cached."""
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        lb, blk, nt, ms = _i(), _i(), _i(), C.c_float()
+        check(lib.einspline_autotune_points(self.ptr, MODES.index(mode),
+                                            pts.ctypes.data_as(C.POINTER(C.c_double)), len(pts),
+                                            C.byref(lb), C.byref(blk), C.byref(nt), C.byref(ms)),
+              "einspline_autotune_points")
         return lb.value, blk.value, nt.value, ms.value
 
     def out_sizes(self, mode, packed=False):

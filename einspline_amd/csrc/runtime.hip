@@ -463,40 +463,12 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
                        void *v, void *g, void *h, int packed, void *stream, void *gh = nullptr);
 static int result_reals(const multi_UBspline_3d_hip *t);
 
-extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample, int *lb_out,
-                                  int *blk_out, int *nt_out, float *ms_out) {
-  // nt_out receives the chosen store policy
-  if (!t || mode < 0 || mode > 3) return fail(EINSPLINE_EINVAL, "bad autotune arguments");
-  if (t->dim != 3) return fail(EINSPLINE_EINVAL, "the autotuner takes 3-D tables");
-  if (sample <= 0) sample = 8192;
+// Time every candidate (tile x store policy x point order, and the
+// matrix-core kernel where it applies) of `mode` on `sample` host positions,
+// apply the fastest to the table and return it.  Synchronises the device.
+static int tune_on(multi_UBspline_3d_hip *t, int mode, const double *hpos, int sample, TuneVal &won,
+                   float &best_out) {
   const int eb = (int)storage_bytes(t->storage);
-  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb, sample};
-  {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    tune_load();
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      einspline_set_tile(t, it->second.lane_bytes, it->second.block);
-      einspline_set_store_policy(t, it->second.store);
-      einspline_set_point_order(t, it->second.order);
-      einspline_set_kernel(t, it->second.kernel);
-      if (lb_out) *lb_out = it->second.lane_bytes;
-      if (blk_out) *blk_out = it->second.block;
-      if (nt_out) *nt_out = it->second.store;
-      if (ms_out) *ms_out = -1.0f;
-      return EINSPLINE_OK;
-    }
-  }
-  // sample points: deterministic, uniform in the box
-  std::vector<double> pos(3 * (size_t)sample);
-  uint64_t x = 0x9e3779b97f4a7c15ull;
-  const Ugrid *gr[3] = {&t->x_grid, &t->y_grid, &t->z_grid};
-  for (size_t i = 0; i < pos.size(); ++i) {
-    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-    const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0);
-    const Ugrid *g = gr[i % 3];
-    pos[i] = g->start + u * (g->end - g->start) * (1.0 - 1e-12);
-  }
   const size_t nreal = (size_t)result_reals(t);
   const size_t es = t->storage == EINSPLINE_STORE_F64 ? 8 : 4;
   void *dp = nullptr, *dv = nullptr, *dg = nullptr, *dh = nullptr;
@@ -504,11 +476,11 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   hipStream_t st = nullptr;
   int rc = EINSPLINE_OK;
   hipError_t e;
-  if ((e = hipMalloc(&dp, pos.size() * 8)) != hipSuccess ||
+  if ((e = hipMalloc(&dp, 3 * (size_t)sample * 8)) != hipSuccess ||
       (e = hipMalloc(&dv, sample * nreal * es)) != hipSuccess ||
       (e = hipMalloc(&dg, 3 * sample * nreal * es)) != hipSuccess ||
       (e = hipMalloc(&dh, 9 * sample * nreal * es)) != hipSuccess ||
-      (e = hipMemcpy(dp, pos.data(), pos.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dp, hpos, 3 * (size_t)sample * 8, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
     rc = hip_fail(e, "autotune setup");
@@ -614,14 +586,73 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   einspline_set_store_policy(t, best_nt);
   einspline_set_point_order(t, best_order);
   einspline_set_kernel(t, best_kernel);
+  won = {best_lb, best_blk, best_nt, best_order, best_kernel};
+  best_out = best;
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample, int *lb_out,
+                                  int *blk_out, int *nt_out, float *ms_out) {
+  // nt_out receives the chosen store policy
+  if (!t || mode < 0 || mode > 3) return fail(EINSPLINE_EINVAL, "bad autotune arguments");
+  if (t->dim != 3) return fail(EINSPLINE_EINVAL, "the autotuner takes 3-D tables");
+  if (sample <= 0) sample = 8192;
+  const int eb = (int)storage_bytes(t->storage);
+  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb, sample};
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = {best_lb, best_blk, best_nt, best_order, best_kernel};
+    tune_load();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      einspline_set_tile(t, it->second.lane_bytes, it->second.block);
+      einspline_set_store_policy(t, it->second.store);
+      einspline_set_point_order(t, it->second.order);
+      einspline_set_kernel(t, it->second.kernel);
+      if (lb_out) *lb_out = it->second.lane_bytes;
+      if (blk_out) *blk_out = it->second.block;
+      if (nt_out) *nt_out = it->second.store;
+      if (ms_out) *ms_out = -1.0f;
+      return EINSPLINE_OK;
+    }
+  }
+  // sample points: deterministic, uniform in the box
+  std::vector<double> pos(3 * (size_t)sample);
+  uint64_t x = 0x9e3779b97f4a7c15ull;
+  const Ugrid *gr[3] = {&t->x_grid, &t->y_grid, &t->z_grid};
+  for (size_t i = 0; i < pos.size(); ++i) {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    const Ugrid *g = gr[i % 3];
+    pos[i] = g->start + u * (g->end - g->start) * (1.0 - 1e-12);
+  }
+  TuneVal won;
+  float best = -1.0f;
+  const int rc = tune_on(t, mode, pos.data(), sample, won, best);
+  if (rc != EINSPLINE_OK) return rc;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = won;
     tune_save();
   }
-  if (lb_out) *lb_out = best_lb;
-  if (blk_out) *blk_out = best_blk;
-  if (nt_out) *nt_out = best_nt;
+  if (lb_out) *lb_out = won.lane_bytes;
+  if (blk_out) *blk_out = won.block;
+  if (nt_out) *nt_out = won.store;
+  if (ms_out) *ms_out = best;
+  return EINSPLINE_OK;
+}
+
+extern "C" int einspline_autotune_points(multi_UBspline_3d_hip *t, int mode, const double *pos, int npoints,
+                                         int *lb_out, int *blk_out, int *nt_out, float *ms_out) {
+  if (!t || mode < 0 || mode > 3 || !pos || npoints <= 0)
+    return fail(EINSPLINE_EINVAL, "bad autotune_points arguments");
+  if (t->dim != 3) return fail(EINSPLINE_EINVAL, "the autotuner takes 3-D tables");
+  TuneVal won;
+  float best = -1.0f;
+  const int rc = tune_on(t, mode, pos, npoints, won, best);
+  if (rc != EINSPLINE_OK) return rc;
+  if (lb_out) *lb_out = won.lane_bytes;
+  if (blk_out) *blk_out = won.block;
+  if (nt_out) *nt_out = won.store;
   if (ms_out) *ms_out = best;
   return EINSPLINE_OK;
 }

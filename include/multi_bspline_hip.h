@@ -301,6 +301,17 @@ int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_point
                        int *lane_bytes, int *block_threads, int *store_policy,
                        float *best_ms);
 
+/* Autotuner on the caller's own positions (no reference counterpart): the same
+ * candidates as einspline_autotune, timed on `npoints` host positions
+ * pos[3*i + {0,1,2}] (a representative sample of the points the caller will
+ * evaluate), the fastest applied to the table.  Not cached: the best point
+ * order depends on how the points are distributed (points clustered in a few
+ * cells favour the given order, uniform points the locality order).
+ * Synchronises the device. */
+int einspline_autotune_points(multi_UBspline_3d_hip *spline, int mode, const double *pos, int npoints,
+                              int *lane_bytes, int *block_threads, int *store_policy,
+                              float *best_ms);
+
 /* ---- single-point path support ----------------------------------------
  * The single-point eval_multi_UBspline_3d_* functions keep a device mirror
  * of the host table and refresh it after set_*.  A caller that writes

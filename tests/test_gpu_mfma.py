@@ -78,7 +78,7 @@ def full_to_packed(H):
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("N", [5, 40, 128, 200, 512])
+@pytest.mark.parametrize("N", [5, 40, 128, 130, 200, 512])
 @pytest.mark.parametrize("packed", [False, True])
 def test_matrix_kernel_matches_oracle_and_vector_kernel(storage, N, packed, rng):
     M = (9, 10, 11)

@@ -140,3 +140,24 @@ def test_autotune_reports_an_order_and_keeps_results(rng):
             assert normwise(a, b) < TOL["s"]
     with pytest.raises(ea.EinsplineError):
         tab.set_point_order(7)
+
+
+def test_autotune_on_clustered_caller_points(rng):
+    """einspline_autotune_points times the candidates on the caller's own
+    positions: points piled into a few cells (where the locality order makes
+    every CU of an XCD read the same rows at once) still tune to a valid
+    configuration, results keep their bar, and bad arguments are refused."""
+    h, _ = make_pair("s", (24, 24, 24), 128, rng)
+    tab = ea.DeviceTable(h)
+    pts = rng.random((8192, 3)) * 0.1          # about 3 cells per axis
+    before = gpu_eval(tab, "vgh", pts)
+    lb, blk, pol, ms = tab.autotune_points("vgh", pts)
+    assert ms > 0.0 and (lb, blk, pol) == tab.tuning()[:3]
+    assert tab.tuning()[3] in (ea.ORDER_GIVEN, ea.ORDER_LOCALITY)
+    for a, b in zip(gpu_eval(tab, "vgh", pts), before):
+        if tab.kernel == ea.KERNEL_VECTOR:
+            np.testing.assert_array_equal(a, b)
+        else:
+            assert normwise(a, b) < TOL["s"]
+    with pytest.raises(ea.EinsplineError):
+        tab.autotune_points("vgh", np.zeros((0, 3)))
