@@ -176,6 +176,24 @@ def test_no_silent_cpu_fallback_without_a_gpu():
     assert ea.lib.einspline_hip_malloc(16) is None
 
 
+def test_autotune_entry_points_refuse_bad_arguments():
+    """Both tuners check their arguments before any device work: no table,
+    a mode outside v..vgh, no positions or no points are EINSPLINE_EINVAL."""
+    import ctypes as C
+
+    import numpy as np
+
+    import einspline_amd as ea
+    pts = np.zeros((4, 3))
+    pp = pts.ctypes.data_as(C.POINTER(C.c_double))
+    assert ea.lib.einspline_autotune(None, 3, 64, None, None, None, None) == ea.E_INVAL
+    assert ea.lib.einspline_autotune_points(None, 3, pp, 4, None, None, None, None) == ea.E_INVAL
+    fake = ea.multi_UBspline_3d_hip()
+    assert ea.lib.einspline_autotune_points(C.byref(fake), 3, None, 4, None, None, None, None) == ea.E_INVAL
+    assert ea.lib.einspline_autotune_points(C.byref(fake), 3, pp, 0, None, None, None, None) == ea.E_INVAL
+    assert ea.lib.einspline_autotune_points(C.byref(fake), 5, pp, 4, None, None, None, None) == ea.E_INVAL
+
+
 def test_fortran_thunks_link():
     """Every C function the reference's Fortran thunks call
     (src/fmulti_bspline.c:772-901 and the 1-D/2-D ones) is exported, so
