@@ -646,9 +646,12 @@ extern "C" int einspline_autotune_points(multi_UBspline_3d_hip *t, int mode, con
   if (!t || mode < 0 || mode > 3 || !pos || npoints <= 0)
     return fail(EINSPLINE_EINVAL, "bad autotune_points arguments");
   if (t->dim != 3) return fail(EINSPLINE_EINVAL, "the autotuner takes 3-D tables");
+  // the first kTunePointsMax positions stand for the rest (bounds the
+  // tuner's output buffers)
+  constexpr int kTunePointsMax = 65536;
   TuneVal won;
   float best = -1.0f;
-  const int rc = tune_on(t, mode, pos, npoints, won, best);
+  const int rc = tune_on(t, mode, pos, npoints < kTunePointsMax ? npoints : kTunePointsMax, won, best);
   if (rc != EINSPLINE_OK) return rc;
   if (lb_out) *lb_out = won.lane_bytes;
   if (blk_out) *blk_out = won.block;

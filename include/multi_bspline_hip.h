@@ -304,7 +304,7 @@ int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_point
 /* Autotuner on the caller's own positions (no reference counterpart): the same
  * candidates as einspline_autotune, timed on `npoints` host positions
  * pos[3*i + {0,1,2}] (a representative sample of the points the caller will
- * evaluate), the fastest applied to the table.  Not cached: the best point
+ * evaluate; at most the first 65536 are used), the fastest applied to the table.  Not cached: the best point
  * order depends on how the points are distributed (points clustered in a few
  * cells favour the given order, uniform points the locality order).
  * Synchronises the device. */
