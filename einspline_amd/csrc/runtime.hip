@@ -412,11 +412,14 @@ extern "C" int einspline_get_tuning(const multi_UBspline_3d_hip *t, int *lane_by
 
 namespace {
 std::mutex g_tune_mu;
+// The best tile depends on the table's extent as well as its row: a table
+// that fits the Infinity Cache favours other tiles than one that streams
+// from HBM.
 struct TuneKey {
-  int storage, tcode, mode, row_bytes, sample;
+  int storage, tcode, mode, row_bytes, sample, nx, ny, nz;
   bool operator<(const TuneKey &o) const {
-    return std::tie(storage, tcode, mode, row_bytes, sample) <
-           std::tie(o.storage, o.tcode, o.mode, o.row_bytes, o.sample);
+    return std::tie(storage, tcode, mode, row_bytes, sample, nx, ny, nz) <
+           std::tie(o.storage, o.tcode, o.mode, o.row_bytes, o.sample, o.nx, o.ny, o.nz);
   }
 };
 struct TuneVal {
@@ -432,15 +435,16 @@ void tune_load() {
   if (!f) return;
   FILE *fp = fopen(f, "r");
   if (!fp) return;
-  // one entry per line, 10 integers; lines of any other shape (older files
-  // with fewer fields) are skipped
+  // one entry per line, 13 integers (8 of the key, then the choice); lines
+  // of any other shape (older files with fewer fields) are skipped
   char line[256];
   while (fgets(line, sizeof line, fp)) {
     TuneKey k;
     TuneVal v;
     char extra;
-    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %c", &k.storage, &k.tcode, &k.mode, &k.row_bytes,
-               &k.sample, &v.lane_bytes, &v.block, &v.store, &v.order, &v.kernel, &extra) == 10)
+    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %c", &k.storage, &k.tcode, &k.mode,
+               &k.row_bytes, &k.sample, &k.nx, &k.ny, &k.nz, &v.lane_bytes, &v.block, &v.store, &v.order,
+               &v.kernel, &extra) == 13)
       g_tune[k] = v;
   }
   fclose(fp);
@@ -452,9 +456,9 @@ void tune_save() {
   FILE *fp = fopen(f, "w");
   if (!fp) return;
   for (auto &kv : g_tune)
-    fprintf(fp, "%d %d %d %d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
-            kv.first.row_bytes, kv.first.sample, kv.second.lane_bytes, kv.second.block,
-            kv.second.store, kv.second.order, kv.second.kernel);
+    fprintf(fp, "%d %d %d %d %d %d %d %d %d %d %d %d %d\n", kv.first.storage, kv.first.tcode, kv.first.mode,
+            kv.first.row_bytes, kv.first.sample, kv.first.nx, kv.first.ny, kv.first.nz, kv.second.lane_bytes,
+            kv.second.block, kv.second.store, kv.second.order, kv.second.kernel);
   fclose(fp);
 }
 }  // namespace
@@ -598,7 +602,7 @@ extern "C" int einspline_autotune(multi_UBspline_3d_hip *t, int mode, int sample
   if (t->dim != 3) return fail(EINSPLINE_EINVAL, "the autotuner takes 3-D tables");
   if (sample <= 0) sample = 8192;
   const int eb = (int)storage_bytes(t->storage);
-  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb, sample};
+  const TuneKey key{(int)t->storage, (int)t->tcode, mode, t->row_reals * eb, sample, t->Nx, t->Ny, t->Nz};
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     tune_load();

@@ -29,6 +29,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 
 #include "runtime_internal.hpp"
@@ -92,6 +93,35 @@ struct Hash {
     return x ^ (x >> 31);
   }
 };
+
+// Every axis a table of dimension `dim` uses needs at least the 4 planes of
+// one stencil, and a grid consistent with its extent: N = num + 3 with
+// delta_inv = num / (end - start) (periodic and antiperiodic conditions), or
+// N = num + 2 with delta_inv = (num - 1) / (end - start) (the others).
+// Axes past `dim` are single planes.  Returns NULL or what is wrong.
+const char *check_axes(const Header &hd) {
+  const int32_t n[3] = {hd.Nx, hd.Ny, hd.Nz};
+  for (int a = 0; a < 3; ++a) {
+    if (a >= hd.dim) {
+      if (n[a] != 1) return "an axis past the table's dimension has more than one plane";
+      continue;
+    }
+    if (n[a] < 4) return "an axis has fewer than 4 planes";
+    const AxisRec &g = hd.axis[a];
+    const double ext = g.end - g.start;
+    // (create accepts end < start: a reversed axis has a negative delta_inv)
+    if (!std::isfinite(g.start) || !std::isfinite(g.end) || !std::isfinite(ext) || ext == 0.0)
+      return "an axis has no extent";
+    if (!std::isfinite(g.delta_inv) || g.delta_inv == 0.0) return "an axis has a bad delta_inv";
+    double intervals;
+    if (n[a] == g.num + 3) intervals = g.num;
+    else if (n[a] == g.num + 2) intervals = g.num - 1;
+    else return "an axis's plane count does not match its grid";
+    if (!(intervals >= 1.0) || fabs(g.delta_inv * ext - intervals) > 1e-9 * intervals)
+      return "an axis's delta_inv does not match its grid";
+  }
+  return nullptr;
+}
 
 uint64_t head_hash(const Header &hd) {
   Hash hs;
@@ -239,12 +269,24 @@ extern "C" multi_UBspline_3d_hip *einspline_table_load(const char *path) {
   if (head_hash(hd) != hd.head_hash) { esk_fail(EINSPLINE_EINVAL, "table_load: %s: header hash mismatch", path); return nullptr; }
   const size_t eb = elem_bytes(hd.storage);
   const bool cplx = hd.tcode == SINGLE_COMPLEX || hd.tcode == DOUBLE_COMPLEX;
-  if (!eb || hd.dim < 1 || hd.dim > 3 || hd.tcode < SINGLE_REAL || hd.tcode > DOUBLE_COMPLEX || hd.Nx < 4 ||
-      hd.Ny < 1 || hd.Nz < 1 || hd.num_splines < 1 ||
+  if (!eb || hd.dim < 1 || hd.dim > 3 || hd.tcode < SINGLE_REAL || hd.tcode > DOUBLE_COMPLEX ||
+      hd.num_splines < 1 || hd.row_reals < 1 ||
       (size_t)hd.row_reals < (size_t)hd.num_splines * (cplx ? 2 : 1) ||
-      (hd.row_reals * eb) % 32 != 0 ||      // the kernels read rows in whole 16- and 32-byte pieces
-      hd.coef_bytes != (uint64_t)hd.Nx * hd.Ny * hd.Nz * hd.row_reals * eb) {
+      ((size_t)hd.row_reals * eb) % 32 != 0) {   // the kernels read rows in whole 16- and 32-byte pieces
     esk_fail(EINSPLINE_EINVAL, "table_load: %s: inconsistent header", path);
+    return nullptr;
+  }
+  const char *bad_axis = check_axes(hd);
+  if (bad_axis) {
+    esk_fail(EINSPLINE_EINVAL, "table_load: %s: inconsistent header: %s", path, bad_axis);
+    return nullptr;
+  }
+  uint64_t want = 0;
+  if (__builtin_mul_overflow((uint64_t)hd.Nx, (uint64_t)hd.Ny, &want) ||
+      __builtin_mul_overflow(want, (uint64_t)hd.Nz, &want) ||
+      __builtin_mul_overflow(want, (uint64_t)hd.row_reals, &want) ||
+      __builtin_mul_overflow(want, (uint64_t)eb, &want) || hd.coef_bytes != want) {
+    esk_fail(EINSPLINE_EINVAL, "table_load: %s: inconsistent header: coefficient bytes", path);
     return nullptr;
   }
   const off_t fsize = lseek(fd, 0, SEEK_END);

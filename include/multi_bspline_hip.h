@@ -295,7 +295,7 @@ int einspline_get_tuning(const multi_UBspline_3d_hip *spline, int *lane_bytes, i
  * fastest with einspline_set_tile / einspline_set_store_policy /
  * einspline_set_point_order and returns the tile and policy (the order is read
  * back with einspline_get_tuning).  Results are cached per
- * (storage, type, mode, row bytes, sample size) for the process, and in the text file
+ * (storage, type, mode, row bytes, sample size, Nx, Ny, Nz) for the process, and in the text file
  * named by $EINSPLINE_TUNE_CACHE when set.  Synchronises the device. */
 int einspline_autotune(multi_UBspline_3d_hip *spline, int mode, int sample_points,
                        int *lane_bytes, int *block_threads, int *store_policy,

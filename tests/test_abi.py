@@ -194,18 +194,6 @@ def test_autotune_entry_points_refuse_bad_arguments():
     assert ea.lib.einspline_autotune_points(C.byref(fake), 5, pp, 4, None, None, None, None) == ea.E_INVAL
 
 
-def test_fortran_thunks_link():
-    """Every C function the reference's Fortran thunks call
-    (src/fmulti_bspline.c:772-901 and the 1-D/2-D ones) is exported, so
-    fmulti_bspline.c rebuilt against include/ links unchanged."""
-    names = open(os.path.join(ROOT, "tests", "golden", "fortran_thunk_calls.txt")).read().split()
-    assert len(names) > 60
-    lib = os.path.join(ROOT, "einspline_amd", "libeinspline.so")
-    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True).stdout
-    exported = {l.split()[-1] for l in out.splitlines()}
-    assert not sorted(set(names) - exported)
-
-
 def test_every_declared_function_has_a_ctypes_prototype():
     """ctypes passes a Python int to an unprototyped function as a C int
     masked to 32 bits, which truncates device addresses: every function of
