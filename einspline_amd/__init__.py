@@ -57,6 +57,7 @@ SINGLE_REAL, DOUBLE_REAL, SINGLE_COMPLEX, DOUBLE_COMPLEX = range(4)
 MULTI_U1D, MULTI_U2D, MULTI_U3D = 6, 7, 8
 STORE_F32, STORE_F64, STORE_BF16, STORE_F16 = range(4)
 MODES = ("v", "vg", "vgl", "vgh")          # autotuner modes 0..3; "vghgh" is batch-only
+NOALLOC_MAX_POINTS = 1 << 20               # EINSPLINE_NOALLOC_MAX_POINTS
 ORDER_GIVEN, ORDER_LOCALITY = 0, 1
 KERNEL_VECTOR, KERNEL_MATRIX = 0, 1
 
@@ -225,6 +226,8 @@ _proto("einspline_eval_v_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, 
 _proto("einspline_eval_vg_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp)
 _proto("einspline_eval_vgl_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp)
 _proto("einspline_eval_vgh_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _i, _vp)
+_proto("einspline_eval_vgh_batch_noalloc", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _i,
+       _vp)
 _proto("einspline_eval_vghgh_batch", _i, C.POINTER(multi_UBspline_3d_hip), _vp, _i, _vp, _vp, _vp, _vp, _vp)
 _proto("einspline_hip_device_count", _i)
 _proto("einspline_hip_last_error", C.c_char_p)

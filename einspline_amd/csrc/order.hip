@@ -162,6 +162,37 @@ void esk_order_free(EskOrderWs *w) {
   delete w;
 }
 
+namespace {
+
+// The two sorting launches on caller-provided buffers: counts (nbins ints,
+// zero on entry), next_counts (cleared for a later call), bin, rank and
+// perm (num ints each).
+hipError_t sort_launch(const esk::Params &P, const void *pos, bool pos_f32, int num, hipStream_t s,
+                       int *counts, int *next_counts, int *bin, int *rank, int *pm) {
+  // key: Morton code of the cell (30 bits at most; coarser cells on huge grids),
+  // then the top bin_bits of it
+  const int bmax = std::max(bits_for(P.nx4), std::max(bits_for(P.ny4), bits_for(P.nz4)));
+  const int cell_shift = bmax > 10 ? bmax - 10 : 0;
+  const int key_bits = 3 * (bmax - cell_shift);
+  int want = 0;
+  while ((1 << (want + 1)) <= num) ++want;                 // about two points per bin
+  const int bin_bits = std::max(0, std::min(std::min(want, key_bits), kMaxBinBits));
+  const int bin_shift = key_bits - bin_bits;
+  const int nbins = 1 << bin_bits;
+  const int blocks = (num + kThreads - 1) / kThreads;
+  if (pos_f32)
+    hipLaunchKernelGGL(bin_points<float>, dim3(blocks), dim3(kThreads), 0, s, (const float *)pos, num, P,
+                       cell_shift, bin_shift, counts, bin, rank);
+  else
+    hipLaunchKernelGGL(bin_points<double>, dim3(blocks), dim3(kThreads), 0, s, (const double *)pos, num, P,
+                       cell_shift, bin_shift, counts, bin, rank);
+  hipLaunchKernelGGL(scan_scatter, dim3((unsigned)((num + kScanThreads - 1) / kScanThreads)), dim3(kScanThreads), 0,
+                     s, counts, next_counts, nbins, bin, rank, num, pm);
+  return hipGetLastError();
+}
+
+}  // namespace
+
 int esk_order_begin(EskOrderWs **slot, const esk::Params &P, const void *pos, bool pos_f32,
                     int num, hipStream_t s, const int **perm) {
   hipError_t e;
@@ -205,30 +236,70 @@ int esk_order_begin(EskOrderWs **slot, const esk::Params &P, const void *pos, bo
   }
   int *counts = w->buf + (size_t)w->parity * kBins, *next_counts = w->buf + (size_t)(1 - w->parity) * kBins;
   int *bin = w->buf + kHead, *rank = bin + w->cap, *pm = rank + w->cap;
-  // key: Morton code of the cell (30 bits at most; coarser cells on huge grids),
-  // then the top bin_bits of it
-  const int bmax = std::max(bits_for(P.nx4), std::max(bits_for(P.ny4), bits_for(P.nz4)));
-  const int cell_shift = bmax > 10 ? bmax - 10 : 0;
-  const int key_bits = 3 * (bmax - cell_shift);
-  int want = 0;
-  while ((1 << (want + 1)) <= num) ++want;                 // about two points per bin
-  const int bin_bits = std::max(0, std::min(std::min(want, key_bits), kMaxBinBits));
-  const int bin_shift = key_bits - bin_bits;
-  const int nbins = 1 << bin_bits;
-  const int blocks = (num + kThreads - 1) / kThreads;
-  if (pos_f32)
-    hipLaunchKernelGGL(bin_points<float>, dim3(blocks), dim3(kThreads), 0, s, (const float *)pos, num, P,
-                       cell_shift, bin_shift, counts, bin, rank);
-  else
-    hipLaunchKernelGGL(bin_points<double>, dim3(blocks), dim3(kThreads), 0, s, (const double *)pos, num, P,
-                       cell_shift, bin_shift, counts, bin, rank);
-  hipLaunchKernelGGL(scan_scatter, dim3((unsigned)((num + kScanThreads - 1) / kScanThreads)), dim3(kScanThreads), 0,
-                     s, counts, next_counts, nbins, bin, rank, num, pm);
-  if ((e = hipGetLastError()) != hipSuccess) {
+  if ((e = sort_launch(P, pos, pos_f32, num, s, counts, next_counts, bin, rank, pm)) != hipSuccess) {
     w->cap = 0;                 // the counts may be dirty: reallocate and clear on the next call
     return bail(esk_hip_fail(e, "locality order launch"));
   }
   w->parity = 1 - w->parity;
+  *perm = pm;
+  return EINSPLINE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free sorting: the same two launches on a file-scope device
+// array instead of a table's workspace, for callers that cannot allocate.
+// Layout: counts[kBins], spare counts[kBins], then bin, rank and perm for
+// at most EINSPLINE_NOALLOC_MAX_POINTS points each (12.6 MB per GPU, held
+// until the process ends).  Every call on a GPU shares that GPU's copy, so
+// calls that overlap on one GPU race; each call clears the counts itself.
+
+constexpr size_t kScratchInts = 2 * (size_t)kBins + 3 * (size_t)EINSPLINE_NOALLOC_MAX_POINTS;
+static __device__ __align__(256) int scratch[kScratchInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int *g_scratch_ptr[kMaxDevices];   // device address of `scratch` per device, once looked up
+}  // namespace
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current.  In host code the array's own name is a host-side
+// stand-in, not a device address: only this pointer may reach kernels.
+static int scratch_of_current_device(int **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return esk_hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return esk_fail(EINSPLINE_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return esk_hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return esk_fail(EINSPLINE_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = (int *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return EINSPLINE_OK;
+}
+
+int esk_order_begin_scratch(const esk::Params &P, const void *pos, bool pos_f32, int num, hipStream_t s,
+                            const int **perm) {
+  if (num < 0 || num > EINSPLINE_NOALLOC_MAX_POINTS)
+    return esk_fail(EINSPLINE_EINVAL, "locality order without allocation: %d points, at most %d", num,
+                    EINSPLINE_NOALLOC_MAX_POINTS);
+  int *base = nullptr;
+  int rc = scratch_of_current_device(&base);
+  if (rc) return rc;
+  int *counts = base, *spare = base + kBins;
+  int *bin = base + 2 * (size_t)kBins, *rank = bin + EINSPLINE_NOALLOC_MAX_POINTS,
+      *pm = rank + EINSPLINE_NOALLOC_MAX_POINTS;
+  // the previous call left its counts behind: clear them with a kernel (no
+  // memset node, so the call can also be captured into a graph)
+  hipLaunchKernelGGL(zero_ints, dim3((unsigned)((kBins + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                     counts, kBins);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = sort_launch(P, pos, pos_f32, num, s, counts, spare, bin, rank, pm);
+  if (e != hipSuccess) return esk_hip_fail(e, "locality order launch");
   *perm = pm;
   return EINSPLINE_OK;
 }

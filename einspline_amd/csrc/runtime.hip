@@ -464,7 +464,8 @@ void tune_save() {
 }  // namespace
 
 static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
-                       void *v, void *g, void *h, int packed, void *stream, void *gh = nullptr);
+                       void *v, void *g, void *h, int packed, void *stream, void *gh = nullptr,
+                       bool noalloc = false);
 static int result_reals(const multi_UBspline_3d_hip *t);
 
 // Time every candidate (tile x store policy x point order, and the
@@ -767,7 +768,7 @@ static int run12(const multi_UBspline_3d_hip *t, int mode, int api, const void *
 }
 
 static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *pos, int num,
-                       void *v, void *g, void *h, int packed, void *stream, void *gh) {
+                       void *v, void *g, void *h, int packed, void *stream, void *gh, bool noalloc) {
   if (t && t->dim != 3) {
     if (packed) return fail(EINSPLINE_EINVAL, "packed Hessians are for 3-D tables");
     return run12(t, mode, esk::API_STRIDED, pos, num, v, g, h, 0, (hipStream_t)stream);
@@ -801,8 +802,17 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
   if ((long long)num * P.chunks >= (1ll << 31))
     return fail(EINSPLINE_EINVAL, "batch too large: points * lanes per point must be < 2^31");
   P.hpacked = packed ? 1 : 0;
-  EskOrderWs *ws;
-  if ((rc = order_begin(t, P, pos, false, num, (hipStream_t)stream, &ws))) return rc;
+  EskOrderWs *ws = nullptr;
+  if (noalloc) {
+    // the locality order sorts into the file-scope scratch array instead of
+    // the table's workspace (also under stream capture: only kernels are enqueued)
+    if (((const TableExt *)t)->point_order == EINSPLINE_ORDER_LOCALITY && num >= kOrderMinPoints) {
+      if ((rc = esk_order_begin_scratch(P, pos, false, num, (hipStream_t)stream, &P.perm))) return rc;
+      P.xcd_runs = 1;
+    }
+  } else if ((rc = order_begin(t, P, pos, false, num, (hipStream_t)stream, &ws))) {
+    return rc;
+  }
   hipError_t e = dispatch(t, L, P, (hipStream_t)stream);
   if (ws) esk_order_end(ws, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
@@ -824,6 +834,14 @@ extern "C" int einspline_eval_vgl_batch(const multi_UBspline_3d_hip *t, const do
 extern "C" int einspline_eval_vgh_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
                                         void *v, void *g, void *h, int packed, void *stream) {
   return run_strided(t, esk::MODE_VGH, pos, num, v, g, h, packed, stream);
+}
+extern "C" int einspline_eval_vgh_batch_noalloc(const multi_UBspline_3d_hip *t, const double *pos, int num,
+                                                void *v, void *g, void *h, int packed, void *stream) {
+  if (num > EINSPLINE_NOALLOC_MAX_POINTS)
+    return fail(EINSPLINE_EINVAL, "einspline_eval_vgh_batch_noalloc: %d points, at most %d", num,
+                EINSPLINE_NOALLOC_MAX_POINTS);
+  if (t && t->dim != 3) return fail(EINSPLINE_EINVAL, "einspline_eval_vgh_batch_noalloc takes 3-D tables");
+  return run_strided(t, esk::MODE_VGH, pos, num, v, g, h, packed, stream, nullptr, true);
 }
 extern "C" int einspline_eval_vghgh_batch(const multi_UBspline_3d_hip *t, const double *pos, int num,
                                           void *v, void *g, void *h, void *gh, void *stream) {

@@ -21,3 +21,7 @@ void esk_order_free(EskOrderWs *w);
 int esk_order_begin(EskOrderWs **slot, const esk::Params &P, const void *pos, bool pos_f32, int num,
                     hipStream_t s, const int **perm);
 void esk_order_end(EskOrderWs *w, hipStream_t s);
+// The same sort into a file-scope device array (no allocation, at most
+// EINSPLINE_NOALLOC_MAX_POINTS points); shared by every such call on the GPU.
+int esk_order_begin_scratch(const esk::Params &P, const void *pos, bool pos_f32, int num, hipStream_t s,
+                            const int **perm);

@@ -236,6 +236,22 @@ int einspline_eval_vghgh_batch(const multi_UBspline_3d_hip *spline, const double
                                void *vals_d, void *grads_d, void *hess_d, void *gradhess_d,
                                void *stream);
 
+/* Allocation-free vgh (no counterpart in the reference).  For callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments and results as
+ * einspline_eval_vgh_batch on a 3-D table, bit for bit, but no call
+ * allocates: when the table's point order is EINSPLINE_ORDER_LOCALITY, the
+ * sort works in a file-scope device array (12.6 MB per GPU, kept until the
+ * process ends) instead of the table's workspace, and it is also done under
+ * stream capture.  num must be at most EINSPLINE_NOALLOC_MAX_POINTS
+ * (EINSPLINE_EINVAL otherwise).  Every such call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (other streams, other
+ * threads) race and must be ordered by the caller.  Does not synchronise. */
+#define EINSPLINE_NOALLOC_MAX_POINTS (1 << 20)
+int einspline_eval_vgh_batch_noalloc(const multi_UBspline_3d_hip *spline, const double *pos_d, int num,
+                                     void *vals_d, void *grads_d, void *hess_d, int hess_packed,
+                                     void *stream);
+
 /* Tile shape of the batched kernels: bytes of an orbital row one lane loads
  * per stencil row (8, 16 or 32; F64 tables 16 or 32, 16-bit tables 8 or 16)
  * and work-items per block (64..256, a multiple of 64).  0 selects the
