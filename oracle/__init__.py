@@ -41,30 +41,45 @@ class otable(C.Structure):
 
 
 _vp, _i, _d = C.c_void_p, C.c_int, C.c_double
-lib.oracle_table_init.restype = _i
-lib.oracle_table_init.argtypes = [C.POINTER(otable), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]
-for k in "sdcz":
-    f = getattr(lib, f"oracle_set_{k}")
-    f.restype = None
-    f.argtypes = [C.POINTER(otable), _vp, _i, _vp]
-for k in "sd":
-    f = getattr(lib, f"oracle_eval_{k}")
-    f.restype = _i
-    f.argtypes = [C.POINTER(otable), _vp, _i, _d, _d, _d, _vp, _vp, _vp, _vp]
-lib.oracle_eval_batch.restype = _i
-lib.oracle_eval_batch.argtypes = [C.POINTER(otable), _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _i]
-lib.oracle_basis_d.restype = None
-lib.oracle_basis_d.argtypes = [_d, _vp, _vp, _vp]
-lib.oracle_basis3_d.restype = None
-lib.oracle_basis3_d.argtypes = [_vp]
-lib.oracle_max_threads.restype = _i
-for t, ct in (("f", C.c_float), ("d", C.c_double)):
-    f = getattr(lib, f"oracle_update_inverse_{t}")
-    f.restype = None
-    f.argtypes = [_vp, _vp, _vp, _i, C.c_long, _i]
-    f = getattr(lib, f"oracle_det_ratio_{t}")
-    f.restype = ct
-    f.argtypes = [_vp, _vp, _i, C.c_long, _i]
+
+
+def _bind(l):
+    """Declare the prototypes of an oracle library (this build's or a
+    native build of the same oracle.c)."""
+    l.oracle_table_init.restype = _i
+    l.oracle_table_init.argtypes = [C.POINTER(otable), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i]
+    for k in "sdcz":
+        f = getattr(l, f"oracle_set_{k}")
+        f.restype = None
+        f.argtypes = [C.POINTER(otable), _vp, _i, _vp]
+    for k in "sd":
+        f = getattr(l, f"oracle_eval_{k}")
+        f.restype = _i
+        f.argtypes = [C.POINTER(otable), _vp, _i, _d, _d, _d, _vp, _vp, _vp, _vp]
+    l.oracle_eval_batch.restype = _i
+    l.oracle_eval_batch.argtypes = [C.POINTER(otable), _i, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _i]
+    l.oracle_basis_d.restype = None
+    l.oracle_basis_d.argtypes = [_d, _vp, _vp, _vp]
+    l.oracle_basis3_d.restype = None
+    l.oracle_basis3_d.argtypes = [_vp]
+    l.oracle_max_threads.restype = _i
+    for t, ct in (("f", C.c_float), ("d", C.c_double)):
+        f = getattr(l, f"oracle_update_inverse_{t}")
+        f.restype = None
+        f.argtypes = [_vp, _vp, _vp, _i, C.c_long, _i]
+        f = getattr(l, f"oracle_det_ratio_{t}")
+        f.restype = ct
+        f.argtypes = [_vp, _vp, _i, C.c_long, _i]
+    return l
+
+
+lib = _bind(lib)
+
+
+def load_library(path):
+    """Another build of oracle.c (for example the CPU baseline's native
+    -O3 -march=native build), prototyped like the default one."""
+    return _bind(C.CDLL(path))
 
 
 def basis(t: float):
@@ -87,7 +102,8 @@ class Table:
     """Oracle spline table of kind s/d/c/z (numpy-owned coefficients), in
     len(M) = 1, 2 or 3 dimensions."""
 
-    def __init__(self, kind, M, start, end, N, bc=None):
+    def __init__(self, kind, M, start, end, N, bc=None, library=None):
+        self.lib = library or lib
         self.kind = kind
         self.dim = dim = len(M)
         M, start, end = (tuple(v) + (1,) * (3 - dim) for v in (M, start, end))
@@ -107,7 +123,7 @@ class Table:
         m_a = np.array(M, np.int32)
         s_a = np.array(start, float)
         e_a = np.array(end, float)
-        rc_ = lib.oracle_table_init(C.byref(self.t), m_a.ctypes.data, s_a.ctypes.data,
+        rc_ = self.lib.oracle_table_init(C.byref(self.t), m_a.ctypes.data, s_a.ctypes.data,
                                     e_a.ctypes.data, lc.ctypes.data, rc.ctypes.data,
                                     lv.ctypes.data, rv.ctypes.data, int(N), self.ncomp, dim)
         assert rc_ == 0, "invalid oracle table"
@@ -128,7 +144,7 @@ class Table:
         else:
             data = np.ascontiguousarray(data, dtype=self.real)
         assert data.size == int(np.prod(self.M))
-        getattr(lib, f"oracle_set_{self.kind}")(C.byref(self.t), self.coefs.ctypes.data, int(n),
+        getattr(self.lib, f"oracle_set_{self.kind}")(C.byref(self.t), self.coefs.ctypes.data, int(n),
                                                  data.ctypes.data)
 
     def eval_batch(self, mode, pos, nthreads=0):
@@ -147,7 +163,7 @@ class Table:
         h = np.zeros((P, nh * NR), self.real) if nh else None
         gh = np.zeros((P, 27 * NR), self.real) if mode == "vghgh" else None
         ptr = lambda a: a.ctypes.data if a is not None else None
-        bad = lib.oracle_eval_batch(C.byref(self.t), 1 if self.real == np.float64 else 0,
+        bad = self.lib.oracle_eval_batch(C.byref(self.t), 1 if self.real == np.float64 else 0,
                                     self.coefs.ctypes.data, m, pos.ctypes.data, P, v.ctypes.data,
                                     ptr(g), ptr(h), ptr(gh), int(nthreads))
         self.bad = bad

@@ -86,6 +86,53 @@ def test_batch_matches_oracle(kind, mode, N, rng):
         assert normwise(g, r) < TOL[kind], (kind, mode, N)
 
 
+@pytest.mark.parametrize("kind", ["s", "d", "c", "z"])
+def test_reference_test_geometry(kind, rng):
+    """The reference's own multi-vs-single test geometry
+    (src/test_multi.c:707-716): periodic 73 x 91 x 29 grid on the box
+    [3.1, 9.1] x [8.7, 12.7] x [4.5, 9.3], 21 orbitals of random data; every
+    mode through the batched path in both point orders (2048 points, so the
+    locality sort runs), the CUDA-style vgh entry point, and the single-point
+    API at a few points."""
+    M, N = (73, 91, 29), 21
+    start, end = (3.1, 8.7, 4.5), (9.1, 12.7, 9.3)
+    h, o = make_pair(kind, M, N, rng, start, end)
+    tab = ea.DeviceTable(h)
+    pts = points(rng, 2048, start, end)
+    for mode in ("v", "vg", "vgl", "vgh"):
+        ref = o.eval_batch(mode, pts)
+        got = gpu_eval(tab, mode, pts)
+        for g, r in zip(got, ref):
+            assert normwise(g, r) < TOL[kind], (kind, mode)
+        tab.set_point_order(ea.ORDER_LOCALITY)
+        for a, b in zip(gpu_eval(tab, mode, pts), got):
+            np.testing.assert_array_equal(a, b)
+        tab.set_point_order(ea.ORDER_GIVEN)
+    # CUDA-style pointer arrays (packed Hessian; float positions for s/c)
+    P = 256
+    fpos = kind in "sc"
+    sub = pts[:P]
+    used = sub.astype(np.float32).astype(np.float64) if fpos else sub
+    pos = ea.DeviceBuffer.from_array(sub.astype(np.float32) if fpos else sub)
+    nr, es = tab.nreal, np.dtype(tab.result_dtype).itemsize
+    vb, gb, hb = (ea.DeviceBuffer(P * k * nr * es) for k in (1, 3, 6))
+    ptrs = lambda b, k: ea.DeviceBuffer.from_array(np.array([b.ptr + p * k * nr * es for p in range(P)], np.uint64))
+    vp, gp, hp = ptrs(vb, 1), ptrs(gb, 3), ptrs(hb, 6)
+    ea.check(getattr(ea.lib, f"eval_multi_multi_UBspline_3d_{kind}_vgh_hip")(tab.ptr, pos.ptr, vp.ptr, gp.ptr,
+                                                                            hp.ptr, P), "vgh_hip")
+    rv = o.eval_batch("v", used)[0]
+    v = vb.download(tab.result_dtype, (P, nr))
+    if kind in "cz":
+        v = v.view(np.complex64 if kind == "c" else np.complex128)
+    assert normwise(v.reshape(P, N), rv) < TOL[kind]
+    # single-point API
+    for p in range(3):
+        for mode in ("v", "vgh"):
+            ref = o.eval_batch(mode, pts[p:p + 1])
+            for g, r in zip(h.eval(mode, *pts[p]), ref):
+                assert normwise(g[None], r) < TOL[kind], (kind, mode, p)
+
+
 @pytest.mark.parametrize("kind", ["s", "d"])
 def test_nonperiodic_tables(kind, rng):
     bcs = [(ea.NATURAL, ea.NATURAL), (ea.FLAT, ea.DERIV1, 0.0, 0.5), (ea.ANTIPERIODIC, ea.ANTIPERIODIC)]
