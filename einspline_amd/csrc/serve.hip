@@ -30,6 +30,7 @@
 #include "einspline_private.h"
 #include "eval_kernel.hpp"
 #include "runtime_internal.hpp"
+#include "serve_policy.hpp"
 
 namespace {
 
@@ -116,8 +117,18 @@ struct Table {
   multi_UBspline_3d_hip meta{};     // geometry of the device table (valid with pinned)
   uint64_t last_use = 0;
   int inflight = 0;                 // requests of this table on the GPU
+  int queued = 0;                   // requests of this table in the pending queue
+  bool draining = false;            // an absent table waits for this one to go idle
+  clk::time_point drain_since{};    // when it was first asked to drain
   int nsplines = 0;
 };
+
+// A resident table asked to make room for an absent one keeps serving the
+// requests queued for it for at most this long; then it gets no new work,
+// goes idle and is evicted.  Serving each table's queue in slices instead
+// of request by request keeps the number of uploads down when two tables
+// that do not fit together are both in demand.
+constexpr double kDrainSliceS = 2e-3;
 
 struct Req {
   uint64_t id = 0;
@@ -129,6 +140,7 @@ struct Req {
   double latency = -1.0;
   bool done = false;
   bool collect = true;              // false: nobody waits, drop at completion
+  bool taken = false;               // in the batch being formed
 };
 
 struct Slot {
@@ -156,6 +168,7 @@ struct einspline_server {
   std::vector<Slot> slots;
   std::thread worker, uploader;
   std::deque<int> uploads;          // tables waiting for the uploader thread
+  std::vector<multi_UBspline_3d_hip *> frees;   // evicted device tables, freed by the uploader
   std::condition_variable cv_upload;
   bool stop = false;
   bool stop_uploader = false;       // set after the worker has drained
@@ -170,7 +183,7 @@ struct einspline_server {
   void upload_loop();
   void poll_completions();
   bool launch_batch(Slot &s, std::unique_lock<std::mutex> &lk);
-  int admit(int tid, std::unique_lock<std::mutex> &lk, const std::vector<char> &pinned);
+  int admit(int tid, bool may_evict, const std::vector<char> &held, std::vector<char> &drain_now);
   hipGraphExec_t graph_for(Slot &s, unsigned mask);
 };
 
@@ -277,42 +290,46 @@ hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
 }
 
 // Make table `tid` resident.  Returns 1 when resident, 0 when deferred
-// (upload queued or in progress, or no room yet), -1 when it can never fit.
-// Uploads run on the uploader thread, so the worker keeps launching batches
-// for resident tables meanwhile.  Eviction frees idle tables only; it prefers
-// the smallest single table that makes enough room (least to re-upload),
-// then least-recently-used ones.
-int einspline_server::admit(int tid, std::unique_lock<std::mutex> &lk,
-                            const std::vector<char> &pinned) {
-  (void)lk;
+// (upload queued or in progress, or waiting for room).  The choice is
+// esk::plan_admission's: evict the cheapest set of idle tables that makes
+// room, or, when that set holds a table with work in flight (or in the
+// batch being formed: `held`), mark it in `drain_now` so that it is drained
+// (kDrainSliceS) and evicted once idle.  Only the oldest waiting request of
+// a pass may evict or drain (`may_evict`); later ones upload only into free
+// room.  Evicted tables are handed to the uploader thread, which frees them
+// outside the lock; uploads run there too, so the worker keeps launching
+// batches for resident tables meanwhile.
+int einspline_server::admit(int tid, bool may_evict, const std::vector<char> &held,
+                            std::vector<char> &drain_now) {
   Table &t = tables[tid];
   if (t.dev) return 1;
   if (t.loading) return 0;
-  const size_t need = predicted_bytes(t);
-  if (need > budget) return -1;
-  while (used + need > budget) {
-    const size_t shortfall = used + need - budget;
-    int victim = -1;
-    for (int i = 0; i < (int)tables.size(); ++i) {
-      const Table &c = tables[i];
-      if (!c.dev || c.inflight || pinned[i] || c.bytes < shortfall) continue;
-      if (victim < 0 || c.bytes < tables[victim].bytes ||
-          (c.bytes == tables[victim].bytes && c.last_use < tables[victim].last_use))
-        victim = i;
-    }
-    if (victim < 0)
-      for (int i = 0; i < (int)tables.size(); ++i) {
-        const Table &c = tables[i];
-        if (c.dev && !c.inflight && !pinned[i] &&
-            (victim < 0 || c.last_use < tables[victim].last_use))
-          victim = i;
-      }
-    if (victim < 0) return 0;
-    destroy_multi_UBspline_3d_hip(tables[victim].dev);
-    tables[victim].dev = nullptr;
-    used -= tables[victim].bytes;
+  std::vector<esk::AdmitTable> view(tables.size());
+  for (size_t i = 0; i < tables.size(); ++i) {
+    const Table &c = tables[i];
+    view[i].bytes = c.dev ? c.bytes : c.loading ? c.reserved : predicted_bytes(c);
+    view[i].state = c.dev ? 1 : c.loading ? 2 : 0;
+    view[i].busy = c.inflight > 0 || held[i];
+    view[i].pending = c.queued;
+    view[i].last_use = c.last_use;
+  }
+  std::vector<int> victims, drain;
+  const int d = esk::plan_admission(view, tid, used, budget, victims, drain);
+  if (d != esk::ADMIT_UPLOAD && d != esk::ADMIT_DEFER) return 0;
+  if (!may_evict && (!victims.empty() || !drain.empty())) return 0;
+  if (d == esk::ADMIT_DEFER) {
+    for (int i : drain) drain_now[i] = 1;
+    return 0;
+  }
+  for (int i : victims) {
+    Table &v = tables[i];
+    frees.push_back(v.dev);
+    v.dev = nullptr;
+    v.draining = false;
+    used -= v.bytes;
     st.evictions++;
   }
+  const size_t need = predicted_bytes(t);
   used += need;
   t.reserved = need;
   t.loading = true;
@@ -325,7 +342,16 @@ void einspline_server::upload_loop() {
   (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
-    cv_upload.wait(lk, [&] { return stop_uploader || !uploads.empty(); });
+    cv_upload.wait(lk, [&] { return stop_uploader || !uploads.empty() || !frees.empty(); });
+    if (!frees.empty()) {
+      // free evicted tables before allocating for the next upload
+      std::vector<multi_UBspline_3d_hip *> f;
+      f.swap(frees);
+      lk.unlock();
+      for (multi_UBspline_3d_hip *d : f) destroy_multi_UBspline_3d_hip(d);
+      lk.lock();
+      continue;
+    }
     if (uploads.empty()) {
       if (stop_uploader) return;
       continue;
@@ -405,30 +431,46 @@ void einspline_server::poll_completions() {
 
 // Form and launch one batch on slot s from the pending queue.
 bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
-  std::vector<char> pinned(tables.size(), 0);
+  (void)lk;
+  std::vector<char> held(tables.size(), 0), drain_now(tables.size(), 0);
   std::vector<Req *> take;
   int points = 0;
-  // Only this thread removes from `pending`; submitters append.  Work on a
-  // snapshot of the head so admit() may drop the lock safely.
-  const size_t look = std::min(pending.size(), (size_t)cfg.max_batch_requests);
-  std::vector<Req *> head(pending.begin(), pending.begin() + look);
-  for (Req *r : head) {
-    if (points + r->npts > cfg.max_batch_points || (int)take.size() >= cfg.max_batch_requests)
-      continue;
-    pinned[r->table] = 1;
-    const int a = admit(r->table, lk, pinned);
-    if (a <= 0) {
-      pinned[r->table] = 0;
+  const auto now = clk::now();
+  // Pass 1: requests of resident tables in arrival order, over the whole
+  // queue, so a table serves everything queued for it; a table being
+  // drained only until its slice is over.  Only this thread removes from
+  // `pending`; submitters append.
+  for (Req *r : pending) {
+    if ((int)take.size() >= cfg.max_batch_requests || points >= cfg.max_batch_points) break;
+    const Table &t = tables[r->table];
+    if (!t.dev || points + r->npts > cfg.max_batch_points) continue;
+    if (t.draining && std::chrono::duration<double>(now - t.drain_since).count() > kDrainSliceS) {
       st.deferrals++;
       continue;
     }
+    held[r->table] = 1;
+    r->taken = true;
     take.push_back(r);
     points += r->npts;
   }
-  if (!take.empty()) {
-    auto is_taken = [&](Req *r) { return std::find(take.begin(), take.end(), r) != take.end(); };
-    pending.erase(std::remove_if(pending.begin(), pending.begin() + look, is_taken),
-                  pending.begin() + look);
+  if (!take.empty())
+    pending.erase(std::remove_if(pending.begin(), pending.end(), [](Req *r) { return r->taken; }),
+                  pending.end());
+  // Pass 2: admission.  The oldest request whose table is absent may evict
+  // or drain; later ones only upload into free room.
+  bool planned = false;
+  std::vector<char> seen(tables.size(), 0);
+  for (Req *r : pending) {
+    if (tables[r->table].dev || seen[r->table]) continue;
+    seen[r->table] = 1;
+    (void)admit(r->table, !planned, held, drain_now);
+    planned = true;
+    st.deferrals++;
+  }
+  for (size_t i = 0; i < tables.size(); ++i) {
+    Table &t = tables[i];
+    if (drain_now[i] && !t.draining) t.drain_since = now;
+    t.draining = drain_now[i] != 0;
   }
   if (take.empty()) return false;
   // group segments by kernel variant
@@ -467,6 +509,7 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     sg.P.ld_g = 3l * sg.P.nreal;
     sg.P.ld_h = (cfg.hess_packed ? 6l : 9l) * sg.P.nreal;
     sg.P.hpacked = cfg.hess_packed ? 1 : 0;
+    tables[r->table].queued--;
     sg.wave0 = B->total_waves[g];
     sg.waves = (int)(((long)r->npts * sg.P.chunks + 63) / 64);
     B->total_waves[g] += sg.waves;
@@ -553,6 +596,7 @@ extern "C" void einspline_server_destroy(einspline_server *s) {
     if (sl.hbuf) (void)hipHostFree(sl.hbuf);
     if (sl.dbuf) (void)hipFree(sl.dbuf);
   }
+  for (multi_UBspline_3d_hip *d : s->frees) destroy_multi_UBspline_3d_hip(d);
   for (Table &t : s->tables) {
     if (t.dev) destroy_multi_UBspline_3d_hip(t.dev);
     if (t.pinned) (void)hipHostFree(t.pinned);
@@ -609,6 +653,7 @@ static int submit_impl(einspline_server *s, int tid, const double *pos, int npts
     r->t0 = clk::now();
     s->live[r->id] = r;
     s->pending.push_back(r);
+    s->tables[tid].queued++;
     s->st.submitted++;
     if (ticket) *ticket = r->id;
   }
@@ -676,6 +721,31 @@ extern "C" int einspline_server_table_resident(einspline_server *s, int tid) {
   std::lock_guard<std::mutex> g(s->mu);
   if (tid < 0 || tid >= (int)s->tables.size()) return esk_fail(EINSPLINE_EINVAL, "unknown table");
   return s->tables[tid].dev != nullptr;
+}
+
+extern "C" int einspline_server_plan_admission(const einspline_admission_table *tables, int ntables,
+                                               int table_id, size_t used, size_t budget, int *victims,
+                                               int *nvictims, int *drain, int *ndrain) {
+  if (!tables || ntables < 1 || table_id < 0 || table_id >= ntables || !victims || !nvictims || !drain ||
+      !ndrain)
+    return esk_fail(EINSPLINE_EINVAL, "plan_admission: bad arguments");
+  std::vector<esk::AdmitTable> view(ntables);
+  for (int i = 0; i < ntables; ++i) {
+    if (tables[i].state < 0 || tables[i].state > 2)
+      return esk_fail(EINSPLINE_EINVAL, "plan_admission: table %d has state %d", i, tables[i].state);
+    view[i] = {tables[i].bytes, tables[i].state, tables[i].busy != 0, tables[i].pending, tables[i].last_use};
+  }
+  std::vector<int> v, d;
+  const int rc = esk::plan_admission(view, table_id, used, budget, v, d);
+  *nvictims = (int)v.size();
+  *ndrain = (int)d.size();
+  std::copy(v.begin(), v.end(), victims);
+  std::copy(d.begin(), d.end(), drain);
+  if (rc == esk::ADMIT_NEVER)
+    return esk_fail(EINSPLINE_EBUDGET, "table of %zu bytes exceeds the budget of %zu bytes", tables[table_id].bytes,
+                    budget);
+  return rc == esk::ADMIT_RESIDENT ? EINSPLINE_ADMIT_RESIDENT
+         : rc == esk::ADMIT_UPLOAD ? EINSPLINE_ADMIT_UPLOAD : EINSPLINE_ADMIT_DEFER;
 }
 
 // ---------------------------------------------------------------------------

@@ -27,7 +27,18 @@ class Load(C.Structure):
                 ("hess_d", C.POINTER(C.c_void_p)), ("box", C.c_double * 3)]
 
 
+class AdmissionTable(C.Structure):
+    _fields_ = [("bytes", C.c_size_t), ("state", C.c_int), ("busy", C.c_int), ("pending", C.c_int),
+                ("last_use", C.c_uint64)]
+
+
+ADMIT_RESIDENT, ADMIT_UPLOAD, ADMIT_DEFER = 0, 1, 2
+
 _vp = C.c_void_p
+lib.einspline_server_plan_admission.restype = C.c_int
+lib.einspline_server_plan_admission.argtypes = [C.POINTER(AdmissionTable), C.c_int, C.c_int, C.c_size_t,
+                                                C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
 lib.einspline_server_create.restype = _vp
 lib.einspline_server_create.argtypes = [C.POINTER(ServerConfig)]
 lib.einspline_server_destroy.restype = None
@@ -50,6 +61,21 @@ lib.einspline_server_table_resident.argtypes = [_vp, C.c_int]
 lib.einspline_server_run_load.restype = C.c_int
 lib.einspline_server_run_load.argtypes = [_vp, C.POINTER(Load), C.POINTER(C.c_double), C.c_size_t,
                                           C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+
+
+def plan_admission(tables, table_id, used, budget):
+    """einspline_server_plan_admission on a list of dicts with keys bytes,
+    state (0 absent, 1 resident, 2 uploading), busy, pending, last_use.
+    Returns (decision or negative error code, victims, drain)."""
+    n = len(tables)
+    arr = (AdmissionTable * n)(*[AdmissionTable(int(t["bytes"]), int(t["state"]), int(t.get("busy", 0)),
+                                                int(t.get("pending", 0)), int(t.get("last_use", 0)))
+                                 for t in tables])
+    v, d = (C.c_int * n)(), (C.c_int * n)()
+    nv, nd = C.c_int(), C.c_int()
+    rc = lib.einspline_server_plan_admission(arr, n, int(table_id), int(used), int(budget), v, C.byref(nv),
+                                             d, C.byref(nd))
+    return rc, sorted(v[:nv.value]), sorted(d[:nd.value])
 
 
 class Server:

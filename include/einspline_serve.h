@@ -11,9 +11,11 @@
  *
  * Semantics
  *  - Tables are registered from host splines and uploaded ("admitted") when
- *    a request needs them and the budget allows; least-recently-used tables
- *    with no work in flight are evicted to make room.  A request whose table
- *    cannot be admitted yet waits until in-flight batches drain.
+ *    a request needs them and the budget allows.  To make room, the set of
+ *    idle tables with the fewest bytes is evicted; when the cheapest set
+ *    includes a table with work in flight, that table gets no new work until
+ *    it is idle (einspline_server_plan_admission).  A request whose table
+ *    cannot be admitted yet waits.
  *  - A request is `npts` points (positions copied at submit) of one table;
  *    results (vgh) go to caller-owned device buffers laid out as in
  *    einspline_eval_vgh_batch.  Each result is bitwise identical to
@@ -84,6 +86,36 @@ int einspline_server_drain(einspline_server *srv);
 int einspline_server_get_stats(einspline_server *srv, einspline_server_stats *out);
 /* 1 if the table is resident on the device now. */
 int einspline_server_table_resident(einspline_server *srv, int table_id);
+
+/* The server's admission decision on its own: no server, no GPU, no clock
+ * (for planning budgets and for testing the policy).  `tables` describes
+ * every registered table; the request needs `table_id` resident and `used`
+ * bytes of the `budget` are taken (resident tables plus uploads in
+ * progress).  Returns
+ *   EINSPLINE_ADMIT_RESIDENT  the table is resident;
+ *   EINSPLINE_ADMIT_UPLOAD    upload it now, after evicting the *nvictims
+ *                             tables listed in victims[] (possibly none):
+ *                             the set of idle tables with the fewest bytes
+ *                             that frees enough room (ties: fewer with
+ *                             queued requests, then least recently used);
+ *   EINSPLINE_ADMIT_DEFER     wait: the table is being uploaded, or the
+ *                             cheapest set holds tables with work in flight,
+ *                             listed in drain[] (*ndrain): they get no new
+ *                             work until they are idle and can be evicted,
+ *                             instead of a larger idle table being evicted;
+ *   EINSPLINE_EBUDGET         the table is larger than the budget.
+ * victims and drain must hold ntables entries. */
+typedef struct {
+  size_t   bytes;       /* device bytes of the table when resident */
+  int      state;       /* 0 absent, 1 resident, 2 being uploaded */
+  int      busy;        /* nonzero: work in flight */
+  int      pending;     /* requests queued for it */
+  uint64_t last_use;    /* logical time of its last batch */
+} einspline_admission_table;
+enum { EINSPLINE_ADMIT_RESIDENT = 0, EINSPLINE_ADMIT_UPLOAD = 1, EINSPLINE_ADMIT_DEFER = 2 };
+int einspline_server_plan_admission(const einspline_admission_table *tables, int ntables, int table_id,
+                                    size_t used, size_t budget, int *victims, int *nvictims, int *drain,
+                                    int *ndrain);
 
 /* Benchmark helper: generate requests on the calling thread and collect
  * their latencies.  Arrivals are Poisson at rate_hz (rate_hz <= 0: closed
