@@ -1,21 +1,18 @@
 // nub.hip -- 1-D complex-double multi-splines on non-uniform grids,
 // evaluated on the GPU (src/multi_nubspline_eval_std_z_impl.h:32-113).
 //
-// Per point: the grid's reverse map gives the interval i (bisection over the
-// points of a general grid, the closed forms of the center and log grids),
-// the Cox-de Boor recursion gives the four basis functions and their
-// derivatives from the basis knots, and every orbital is the weighted sum of
-// coefficient rows i..i+3.  One work-item per (point, orbital): a row of
-// orbitals is contiguous, so the four row reads of a wavefront are
-// coalesced.  The grid search and basis are recomputed by every work-item of
-// a point (a few dozen flops, all from L1-resident data).
-//
-// The arithmetic is the host's (nub_create.c) with contraction off, in the
-// same order, so results agree with the reference's double evaluation to the
-// rounding of log / log1p in the center and log reverse maps.  The device
-// table carries one zero row past the last coefficient row: the reference
-// evaluates x >= the last grid point on interval num_points-1, whose fourth
-// row lies past the table (its weight is zero at the last point itself).
+// Per point: the interval from the grid (bisection over the points of a
+// general grid; the closed-form inverse of a center or log grid, corrected
+// against the points), the four basis functions and their derivatives by
+// de Boor's triangular scheme on the basis knots (nub_basis.h, the host's
+// code), then every orbital is the weighted sum of coefficient rows
+// i..i+3.  One work-item per (point, orbital): a row of orbitals is
+// contiguous, so the four row reads of a wavefront are coalesced.  The
+// interval and basis are recomputed by every work-item of a point (a few
+// dozen flops, all from L1-resident data).  The device table carries one
+// zero row past the last coefficient row: a point at or past the last grid
+// point is evaluated on interval num_points-1, whose fourth row lies past
+// the table (its weight is zero at the last point itself).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -25,6 +22,7 @@
 #include <string.h>
 
 #include "einspline_private.h"
+#include "nub_basis.h"
 #include "runtime_internal.hpp"
 
 struct multi_NUBspline_1d_hip {
@@ -36,7 +34,6 @@ struct multi_NUBspline_1d_hip {
   double ainv, startinv;
   double *points;       // device grid points
   double *xvals;        // device basis knots (npts + 5)
-  double *dxinv;        // device 3 (npts + 2)
   // single-point staging: position, then 3 N complex results
   double *stage_d;
   double *stage_h;      // pinned
@@ -49,71 +46,39 @@ struct NubArgs {
   const double *coefs;
   int N, code, npts, half_points;
   double center, aInv, bInv, even_half, ainv, startinv;
-  const double *points, *xvals, *dxinv;
+  const double *points, *xvals;
 };
 
-__device__ int nub_reverse(const NubArgs &g, double x) {
-  const int n = g.npts;
-  double f;
+// floor(f) as an index guess in [0, n], 0 for NaN
+__device__ int guess_of(double f, int n) {
+  if (!(f == f) || f < 0.0) return 0;
+  return f > (double)n ? n : (int)f;
+}
+
+__device__ int nub_interval(const NubArgs &g, double x) {
+  int guess = -1;
   if (g.code == CENTER) {
     const double u = x - g.center;
-    const double k = copysign(log1p(fabs(u) * g.aInv) * g.bInv, u);
-    f = floor(g.half_points + k - g.even_half);
+    guess = guess_of(floor(copysign(log1p(fabs(u) * g.aInv) * g.bInv, u) + g.half_points - g.even_half), g.npts);
   } else if (g.code == LOG) {
-    f = floor(g.ainv * log(x * g.startinv));
-  } else {
-    const double *p = g.points;
-    if (x <= p[0]) return 0;
-    if (x >= p[n - 1]) return n - 1;
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (p[mid] > x) hi = mid; else lo = mid;
-    }
-    return lo;
+    guess = guess_of(floor(x > 0.0 ? log(x * g.startinv) * g.ainv : -1.0), g.npts);
   }
-  if (!(f == f)) return 0;
-  const int i = f < -1.0 ? -1 : (f > (double)n ? n : (int)f);
-  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+  return esk_nub_interval(g.points, g.npts, x, guess);
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(256) nub1d_eval(NubArgs g, const double *__restrict__ pos, int num,
                                                   double2 *__restrict__ vals, double2 *__restrict__ grads,
                                                   double2 *__restrict__ lapl) {
-#pragma clang fp contract(off)
   const long gid = (long)blockIdx.x * 256 + threadIdx.x;
   const long p = gid / g.N;
   const int n = (int)(gid - p * g.N);
   if (p >= num) return;
   const double x = pos[p];
-  const int i = nub_reverse(g, x);
-  const double *dx = g.dxinv, *xv = g.xvals;
-  const int i2 = i + 2;
-  double b1[2], b2[3], a[4], da[4], d2a[4];
-  b1[0] = (xv[i2 + 1] - x) * dx[3 * (i + 2) + 0];
-  b1[1] = (x - xv[i2]) * dx[3 * (i + 2) + 0];
-  b2[0] = (xv[i2 + 1] - x) * dx[3 * (i + 1) + 1] * b1[0];
-  b2[1] = ((x - xv[i2 - 1]) * dx[3 * (i + 1) + 1] * b1[0] + (xv[i2 + 2] - x) * dx[3 * (i + 2) + 1] * b1[1]);
-  b2[2] = (x - xv[i2]) * dx[3 * (i + 2) + 1] * b1[1];
-  a[0] = (xv[i2 + 1] - x) * dx[3 * (i) + 2] * b2[0];
-  a[1] = ((x - xv[i2 - 2]) * dx[3 * (i) + 2] * b2[0] + (xv[i2 + 2] - x) * dx[3 * (i + 1) + 2] * b2[1]);
-  a[2] = ((x - xv[i2 - 1]) * dx[3 * (i + 1) + 2] * b2[1] + (xv[i2 + 3] - x) * dx[3 * (i + 2) + 2] * b2[2]);
-  a[3] = (x - xv[i2]) * dx[3 * (i + 2) + 2] * b2[2];
-  if (MODE >= 1) {
-    da[0] = -3.0 * (dx[3 * (i) + 2] * b2[0]);
-    da[1] = 3.0 * (dx[3 * (i) + 2] * b2[0] - dx[3 * (i + 1) + 2] * b2[1]);
-    da[2] = 3.0 * (dx[3 * (i + 1) + 2] * b2[1] - dx[3 * (i + 2) + 2] * b2[2]);
-    da[3] = 3.0 * (dx[3 * (i + 2) + 2] * b2[2]);
-  }
-  if (MODE >= 2) {
-    d2a[0] = 6.0 * (+dx[3 * (i + 0) + 2] * dx[3 * (i + 1) + 1] * b1[0]);
-    d2a[1] = 6.0 * (-dx[3 * (i + 1) + 1] * (dx[3 * (i + 0) + 2] + dx[3 * (i + 1) + 2]) * b1[0] +
-                    dx[3 * (i + 1) + 2] * dx[3 * (i + 2) + 1] * b1[1]);
-    d2a[2] = 6.0 * (+dx[3 * (i + 1) + 2] * dx[3 * (i + 1) + 1] * b1[0] -
-                    dx[3 * (i + 2) + 1] * (dx[3 * (i + 1) + 2] + dx[3 * (i + 2) + 2]) * b1[1]);
-    d2a[3] = 6.0 * (+dx[3 * (i + 2) + 2] * dx[3 * (i + 2) + 1] * b1[1]);
-  }
+  const int i = nub_interval(g, x);
+  double w[3][4];
+  esk_nub_basis(g.xvals, i, x, MODE, w);
+  const double *a = w[0], *da = w[1], *d2a = w[2];
   const double2 *c = (const double2 *)g.coefs + (long)i * g.N + n;
   double2 r[4];
 #pragma unroll
@@ -146,7 +111,7 @@ NubArgs args_of(const multi_NUBspline_1d_hip *t) {
   g.N = t->N; g.code = t->code; g.npts = t->npts; g.half_points = t->half_points;
   g.center = t->center; g.aInv = t->aInv; g.bInv = t->bInv; g.even_half = t->even_half;
   g.ainv = t->ainv; g.startinv = t->startinv;
-  g.points = t->points; g.xvals = t->xvals; g.dxinv = t->dxinv;
+  g.points = t->points; g.xvals = t->xvals;
   return g;
 }
 
@@ -170,7 +135,6 @@ void free_table(multi_NUBspline_1d_hip *t) {
   (void)hipFree(t->coefs);
   (void)hipFree(t->points);
   (void)hipFree(t->xvals);
-  (void)hipFree(t->dxinv);
   (void)hipFree(t->stage_d);
   if (t->stage_h) (void)hipHostFree(t->stage_h);
   if (t->stream) (void)hipStreamDestroy(t->stream);
@@ -204,15 +168,13 @@ extern "C" multi_NUBspline_1d_hip *create_multi_NUBspline_1d_z_hip(multi_NUBspli
   if ((e = hipMalloc((void **)&t->coefs, cbytes)) != hipSuccess ||
       (e = hipMalloc((void **)&t->points, sizeof(double) * t->npts)) != hipSuccess ||
       (e = hipMalloc((void **)&t->xvals, sizeof(double) * (t->npts + 5))) != hipSuccess ||
-      (e = hipMalloc((void **)&t->dxinv, sizeof(double) * 3 * (t->npts + 2))) != hipSuccess ||
       (e = hipMalloc((void **)&t->stage_d, 256 + 3 * rowb)) != hipSuccess ||
       (e = hipHostMalloc((void **)&t->stage_h, 256 + 3 * rowb, hipHostMallocDefault)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipMemcpy(t->coefs, spline->coefs, (size_t)t->nrows * rowb, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemset((char *)t->coefs + (size_t)t->nrows * rowb, 0, rowb)) != hipSuccess ||
       (e = hipMemcpy(t->points, grid->points, sizeof(double) * t->npts, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(t->xvals, basis->xVals, sizeof(double) * (t->npts + 5), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(t->dxinv, basis->dxInv, sizeof(double) * 3 * (t->npts + 2), hipMemcpyHostToDevice)) != hipSuccess) {
+      (e = hipMemcpy(t->xvals, basis->xVals, sizeof(double) * (t->npts + 5), hipMemcpyHostToDevice)) != hipSuccess) {
     esk_hip_fail(e, "create_multi_NUBspline_1d_z_hip");
     free_table(t);
     return nullptr;

@@ -1,21 +1,21 @@
 /* nub_create.c -- non-uniform grids, their cubic B-spline basis, and the
  * 1-D non-uniform multi-splines (create / set / destroy).
  *
- * Follows the reference's definitions:
- *   grids:  src/nugrid.c:25-172 (general grid: bisection reverse map; center
- *           grid: points c +- a expm1(b k), reverse map by log1p; log grid:
- *           points start exp(a i), reverse map by log);
- *   basis:  src/nubasis.c:24-685 (two extra knots before the grid and three
- *           after; Cox-de Boor recursion over dxInv[3i+j] = 1/(x_{i+j+1}-x_i));
- *   solves: src/nubspline_create.c:389-574 (tridiagonal elimination with the
- *           two derivative rows, or the cyclic system with a last column);
- *   multi:  src/multi_nubspline_create.c:643-700, :898-965 (one orbital per
- *           set call, complex orbitals as two real solves).
- * The steps of the recursion and of the eliminations are the reference's,
- * in the same order, so results match it to the last bit in double.  Unlike
- * the reference, every reverse map clamps its index to [0, num_points-1] (the
- * center and log maps can otherwise leave the table for points outside the
- * grid).
+ * Interfaces and definitions follow the reference (src/nugrid.h, nubasis.h,
+ * multi_nubspline_create.h): the same grid point formulas, the same knot
+ * sequence (grid points with two knots before and three after, the end
+ * spacings repeated, or wrapped for periodic bases), the same coefficient
+ * layout and boundary conditions.  The computations are this library's:
+ *   - reverse maps: a closed-form first guess for center and log grids,
+ *     corrected against the stored points (so the interval brackets x even
+ *     where log / log1p round), bisection on general grids;
+ *   - basis: de Boor's triangular scheme (nub_basis.h), knot differences
+ *     formed on the fly;
+ *   - interpolation: the collocation system is factored once per set call
+ *     (Thomas factors of its tridiagonal part after folding the derivative
+ *     rows' corner entries; periodic systems by bordering the last unknown)
+ *     and applied to every real component of the data.
+ * Results agree with the reference's to rounding, not bit for bit.
  */
 #include <math.h>
 #include <pthread.h>
@@ -24,113 +24,103 @@
 #include <string.h>
 
 #include "einspline_private.h"
+#include "nub_basis.h"
 
 /* ------------------------------------------------------------------------ */
 /* grids                                                                     */
 
-static int clamp_index(int i, int n) { return i < 0 ? 0 : (i > n - 1 ? n - 1 : i); }
-
-static int general_reverse(void *gp, double x) {
+static int general_interval(void *gp, double x) {
   const NUgrid *g = (const NUgrid *)gp;
-  const int n = g->num_points;
-  const double *p = g->points;
-  if (x <= p[0]) return 0;
-  if (x >= p[n - 1]) return n - 1;
-  int lo = 0, hi = n - 1;                 /* p[lo] <= x < p[hi] */
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (p[mid] > x) hi = mid; else lo = mid;
-  }
-  return lo;
+  return esk_nub_interval(g->points, g->num_points, x, -1);
 }
 
-static int center_reverse(void *gp, double x) {
+/* floor(f) as an index guess in [0, n], 0 for NaN */
+static int guess_of(double f, int n) {
+  if (!(f == f) || f < 0.0) return 0;
+  return f > (double)n ? n : (int)f;
+}
+
+static int center_interval(void *gp, double x) {
   const center_grid *g = (const center_grid *)gp;
+  /* invert x = center +- a expm1(b |k - half_points + even_half|) */
   const double u = x - g->center;
-  const double k = copysign(log1p(fabs(u) * g->aInv) * g->bInv, u);
-  const double f = floor(g->half_points + k - g->even_half);
-  if (!(f == f)) return 0;
-  return clamp_index(f < -1.0 ? -1 : (f > (double)g->num_points ? g->num_points : (int)f), g->num_points);
+  const double k = copysign(log1p(fabs(u) * g->aInv) * g->bInv, u) + g->half_points - g->even_half;
+  return esk_nub_interval(g->points, g->num_points, x, guess_of(floor(k), g->num_points));
 }
 
-static int log_reverse(void *gp, double x) {
+static int log_interval(void *gp, double x) {
   const log_grid *g = (const log_grid *)gp;
-  const double f = floor(g->ainv * log(x * g->startinv));
-  if (!(f == f)) return 0;
-  return clamp_index(f < -1.0 ? -1 : (f > (double)g->num_points ? g->num_points : (int)f), g->num_points);
+  /* invert x = start exp(a i) */
+  const double k = x > 0.0 ? log(x * g->startinv) * g->ainv : -1.0;
+  return esk_nub_interval(g->points, g->num_points, x, guess_of(floor(k), g->num_points));
+}
+
+static NUgrid *grid_alloc(size_t bytes, int n) {
+  NUgrid *g = calloc(1, bytes);
+  if (!g) return NULL;
+  g->points = malloc(sizeof(double) * (size_t)n);
+  if (!g->points) { free(g); return NULL; }
+  g->num_points = n;
+  return g;
 }
 
 NUgrid *create_general_grid(double *points, int num_points) {
   if (!points || num_points < 2) return NULL;
-  NUgrid *g = calloc(1, sizeof(NUgrid));
+  for (int i = 1; i < num_points; ++i)
+    if (!(points[i] > points[i - 1])) return NULL;      /* strictly increasing */
+  NUgrid *g = grid_alloc(sizeof(NUgrid), num_points);
   if (!g) return NULL;
-  g->points = malloc(sizeof(double) * (size_t)num_points);
-  if (!g->points) { free(g); return NULL; }
   memcpy(g->points, points, sizeof(double) * (size_t)num_points);
   g->code = GENERAL;
   g->start = points[0];
   g->end = points[num_points - 1];
-  g->num_points = num_points;
-  g->reverse_map = general_reverse;
+  g->reverse_map = general_interval;
   return g;
 }
 
+/* Points at center + sign(k) a expm1(b |k|): k = -h..h for an odd count,
+ * k = -h+1/2 .. h-1/2 for an even one, h = num_points / 2; b = log(ratio) /
+ * (h - 1) and a puts the outermost points at start and end. */
 NUgrid *create_center_grid(double start, double end, double ratio, int num_points) {
-  if (!(ratio > 1.0) || num_points < 4) return NULL;
-  center_grid *g = calloc(1, sizeof(center_grid));
+  if (!(ratio > 1.0) || num_points < 4 || !(end > start)) return NULL;
+  center_grid *g = (center_grid *)grid_alloc(sizeof(center_grid), num_points);
   if (!g) return NULL;
-  g->points = malloc(sizeof(double) * (size_t)num_points);
-  if (!g->points) { free(g); return NULL; }
+  const int h = num_points / 2;
   g->code = CENTER;
   g->start = start;
   g->end = end;
   g->center = 0.5 * (start + end);
-  g->num_points = num_points;
-  g->half_points = num_points / 2;
-  g->odd = (num_points % 2) == 1;
-  g->b = log(ratio) / (double)(g->half_points - 1);
+  g->half_points = h;
+  g->odd = (num_points & 1) != 0;
+  g->odd_one = g->odd ? 1 : 0;
+  g->even_half = g->odd ? 0.0 : 0.5;
+  g->b = log(ratio) / (double)(h - 1);
   g->bInv = 1.0 / g->b;
-  const int h = g->half_points;
-  if (g->odd) {
-    /* points at c + sign(k) a expm1(b |k|), k = -h..h */
-    g->even_half = 0.0;
-    g->odd_one = 1;
-    g->a = 0.5 * (end - start) / expm1(g->b * h);
-    g->aInv = 1.0 / g->a;
-    for (int k = -h; k <= h; ++k) {
-      const double sign = k < 0 ? -1.0 : 1.0;
-      g->points[k + h] = sign * g->a * expm1(g->b * abs(k)) + g->center;
-    }
-  } else {
-    /* points at c + sign(k) a expm1(b |k + 1/2|), k = -h..h-1 */
-    g->even_half = 0.5;
-    g->odd_one = 0;
-    g->a = 0.5 * (end - start) / expm1(g->b * (-0.5 + h));
-    g->aInv = 1.0 / g->a;
-    for (int k = -h; k < h; ++k) {
-      const double sign = k < 0 ? -1.0 : 1.0;
-      g->points[k + h] = sign * g->a * expm1(g->b * fabs(0.5 + k)) + g->center;
-    }
+  const double kmax = (double)h - g->even_half;           /* |k| of the outermost points */
+  g->a = 0.5 * (end - start) / expm1(g->b * kmax);
+  g->aInv = 1.0 / g->a;
+  for (int i = 0; i < num_points; ++i) {
+    const double k = (double)(i - h) + g->even_half;       /* signed position from the centre */
+    const double r = g->a * expm1(g->b * fabs(k));
+    g->points[i] = (k < 0.0 ? -r : r) + g->center;
   }
-  g->reverse_map = center_reverse;
+  g->reverse_map = center_interval;
   return (NUgrid *)g;
 }
 
+/* points[i] = start exp(a i), a = log(end / start) / (num_points - 1). */
 NUgrid *create_log_grid(double start, double end, int num_points) {
   if (!(start > 0.0) || !(end > start) || num_points < 2) return NULL;
-  log_grid *g = calloc(1, sizeof(log_grid));
+  log_grid *g = (log_grid *)grid_alloc(sizeof(log_grid), num_points);
   if (!g) return NULL;
-  g->points = malloc(sizeof(double) * (size_t)num_points);
-  if (!g->points) { free(g); return NULL; }
   g->code = LOG;
   g->start = start;
   g->end = end;
-  g->num_points = num_points;
-  g->a = 1.0 / (double)(num_points - 1) * log(end / start);
+  g->a = log(end / start) / (double)(num_points - 1);
   g->ainv = 1.0 / g->a;
   g->startinv = 1.0 / start;
   for (int i = 0; i < num_points; ++i) g->points[i] = start * exp(g->a * (double)i);
-  g->reverse_map = log_reverse;
+  g->reverse_map = log_interval;
   return (NUgrid *)g;
 }
 
@@ -154,25 +144,17 @@ NUBasis *create_NUBasis(NUgrid *grid, bool periodic) {
   b->dxInv = malloc(sizeof(double) * 3 * (size_t)(N + 2));
   if (!b->xVals || !b->dxInv) { free(b->xVals); free(b->dxInv); free(b); return NULL; }
   const double *g = grid->points;
-  double *xv = b->xVals;
-  for (int i = 0; i < N; ++i) xv[i + 2] = g[i];
-  if (!periodic) {
-    /* extend with the end spacings */
-    xv[0] = g[0] - 2.0 * (g[1] - g[0]);
-    xv[1] = g[0] - 1.0 * (g[1] - g[0]);
-    xv[N + 2] = g[N - 1] + 1.0 * (g[N - 1] - g[N - 2]);
-    xv[N + 3] = g[N - 1] + 2.0 * (g[N - 1] - g[N - 2]);
-    xv[N + 4] = g[N - 1] + 3.0 * (g[N - 1] - g[N - 2]);
-  } else {
-    /* wrap the spacings from the other end */
-    xv[1] = g[0] - (g[N - 1] - g[N - 2]);
-    xv[0] = g[0] - (g[N - 1] - g[N - 3]);
-    xv[N + 2] = g[N - 1] + (g[1] - g[0]);
-    xv[N + 3] = g[N - 1] + (g[2] - g[0]);
-    xv[N + 4] = g[N - 1] + (g[3] - g[0]);
-  }
+  double *kn = b->xVals;
+  memcpy(kn + 2, g, sizeof(double) * (size_t)N);
+  /* two knots before the grid and three after: the end spacing repeated,
+   * or, for a periodic basis, the spacings of the other end */
+  for (int k = 1; k <= 2; ++k)
+    kn[2 - k] = g[0] - (periodic ? g[N - 1] - g[N - 1 - k] : (double)k * (g[1] - g[0]));
+  for (int k = 1; k <= 3; ++k)
+    kn[N + 1 + k] = g[N - 1] + (periodic ? g[k] - g[0] : (double)k * (g[N - 1] - g[N - 2]));
+  /* the documented table of inverse knot spans (not used by the evaluation) */
   for (int i = 0; i < N + 2; ++i)
-    for (int j = 0; j < 3; ++j) b->dxInv[3 * i + j] = 1.0 / (xv[i + j + 1] - xv[i]);
+    for (int j = 0; j < 3; ++j) b->dxInv[3 * i + j] = 1.0 / (kn[i + j + 1] - kn[i]);
   return b;
 }
 
@@ -183,227 +165,247 @@ void destroy_NUBasis(NUBasis *basis) {
   free(basis);
 }
 
-/* The four non-zero cubic basis functions on interval i at x (and their
- * first and second derivatives when d1 / d2 are non-NULL): degree-1 values
- * b1, degree-2 values b2, then degree 3, each level from the one below. */
-static void nub_funcs(const NUBasis *basis, int i, double x, double b[4], double *d1, double *d2) {
-  const double *dx = basis->dxInv, *xv = basis->xVals;
-  const int i2 = i + 2;
-  double b1[2], b2[3];
-  b1[0] = (xv[i2 + 1] - x) * dx[3 * (i + 2) + 0];
-  b1[1] = (x - xv[i2]) * dx[3 * (i + 2) + 0];
-  b2[0] = (xv[i2 + 1] - x) * dx[3 * (i + 1) + 1] * b1[0];
-  b2[1] = ((x - xv[i2 - 1]) * dx[3 * (i + 1) + 1] * b1[0] + (xv[i2 + 2] - x) * dx[3 * (i + 2) + 1] * b1[1]);
-  b2[2] = (x - xv[i2]) * dx[3 * (i + 2) + 1] * b1[1];
-  b[0] = (xv[i2 + 1] - x) * dx[3 * (i) + 2] * b2[0];
-  b[1] = ((x - xv[i2 - 2]) * dx[3 * (i) + 2] * b2[0] + (xv[i2 + 2] - x) * dx[3 * (i + 1) + 2] * b2[1]);
-  b[2] = ((x - xv[i2 - 1]) * dx[3 * (i + 1) + 2] * b2[1] + (xv[i2 + 3] - x) * dx[3 * (i + 2) + 2] * b2[2]);
-  b[3] = (x - xv[i2]) * dx[3 * (i + 2) + 2] * b2[2];
-  if (d1) {
-    d1[0] = -3.0 * (dx[3 * (i) + 2] * b2[0]);
-    d1[1] = 3.0 * (dx[3 * (i) + 2] * b2[0] - dx[3 * (i + 1) + 2] * b2[1]);
-    d1[2] = 3.0 * (dx[3 * (i + 1) + 2] * b2[1] - dx[3 * (i + 2) + 2] * b2[2]);
-    d1[3] = 3.0 * (dx[3 * (i + 2) + 2] * b2[2]);
-  }
-  if (d2) {
-    d2[0] = 6.0 * (+dx[3 * (i + 0) + 2] * dx[3 * (i + 1) + 1] * b1[0]);
-    d2[1] = 6.0 * (-dx[3 * (i + 1) + 1] * (dx[3 * (i + 0) + 2] + dx[3 * (i + 1) + 2]) * b1[0] +
-                   dx[3 * (i + 1) + 2] * dx[3 * (i + 2) + 1] * b1[1]);
-    d2[2] = 6.0 * (+dx[3 * (i + 1) + 2] * dx[3 * (i + 1) + 1] * b1[0] -
-                   dx[3 * (i + 2) + 1] * (dx[3 * (i + 1) + 2] + dx[3 * (i + 2) + 2]) * b1[1]);
-    d2[3] = 6.0 * (+dx[3 * (i + 2) + 2] * dx[3 * (i + 2) + 1] * b1[1]);
-  }
+/* Interval of x and the functions (and derivatives up to nder) there. */
+static int basis_at(const NUBasis *basis, double x, int nder, double out[3][4]) {
+  const int i = basis->grid->reverse_map(basis->grid, x);
+  esk_nub_basis(basis->xVals, i, x, nder, out);
+  return i;
 }
 
-static int nub_index(const NUBasis *basis, double x) {
-  return basis->grid->reverse_map(basis->grid, x);
+static void basis_at_point(const NUBasis *basis, int i, int nder, double out[3][4]) {
+  esk_nub_basis(basis->xVals, i, basis->grid->points[i], nder, out);
+}
+
+static void copy4d(double *dst, const double *src) { memcpy(dst, src, 4 * sizeof(double)); }
+static void copy4f(float *dst, const double *src) {
+  for (int k = 0; k < 4; ++k) dst[k] = (float)src[k];
 }
 
 int get_NUBasis_funcs_d(NUBasis *basis, double x, double bf[4]) {
-  const int i = nub_index(basis, x);
-  nub_funcs(basis, i, x, bf, NULL, NULL);
+  double o[3][4];
+  const int i = basis_at(basis, x, 0, o);
+  copy4d(bf, o[0]);
   return i;
 }
 void get_NUBasis_funcs_di(NUBasis *basis, int i, double bf[4]) {
-  nub_funcs(basis, i, basis->grid->points[i], bf, NULL, NULL);
+  double o[3][4];
+  basis_at_point(basis, i, 0, o);
+  copy4d(bf, o[0]);
 }
 int get_NUBasis_dfuncs_d(NUBasis *basis, double x, double bf[4], double dbf[4]) {
-  const int i = nub_index(basis, x);
-  nub_funcs(basis, i, x, bf, dbf, NULL);
+  double o[3][4];
+  const int i = basis_at(basis, x, 1, o);
+  copy4d(bf, o[0]); copy4d(dbf, o[1]);
   return i;
 }
 void get_NUBasis_dfuncs_di(NUBasis *basis, int i, double bf[4], double dbf[4]) {
-  nub_funcs(basis, i, basis->grid->points[i], bf, dbf, NULL);
+  double o[3][4];
+  basis_at_point(basis, i, 1, o);
+  copy4d(bf, o[0]); copy4d(dbf, o[1]);
 }
 int get_NUBasis_d2funcs_d(NUBasis *basis, double x, double bf[4], double dbf[4], double d2bf[4]) {
-  const int i = nub_index(basis, x);
-  nub_funcs(basis, i, x, bf, dbf, d2bf);
+  double o[3][4];
+  const int i = basis_at(basis, x, 2, o);
+  copy4d(bf, o[0]); copy4d(dbf, o[1]); copy4d(d2bf, o[2]);
   return i;
 }
 void get_NUBasis_d2funcs_di(NUBasis *basis, int i, double bf[4], double dbf[4], double d2bf[4]) {
-  nub_funcs(basis, i, basis->grid->points[i], bf, dbf, d2bf);
+  double o[3][4];
+  basis_at_point(basis, i, 2, o);
+  copy4d(bf, o[0]); copy4d(dbf, o[1]); copy4d(d2bf, o[2]);
 }
 
-static void to_float(const double *a, float *b, int n) {
-  for (int k = 0; k < n; ++k) b[k] = (float)a[k];
-}
 int get_NUBasis_funcs_s(NUBasis *basis, double x, float bf[4]) {
-  double b[4];
-  const int i = get_NUBasis_funcs_d(basis, x, b);
-  to_float(b, bf, 4);
+  double o[3][4];
+  const int i = basis_at(basis, x, 0, o);
+  copy4f(bf, o[0]);
   return i;
 }
 void get_NUBasis_funcs_si(NUBasis *basis, int i, float bf[4]) {
-  double b[4];
-  get_NUBasis_funcs_di(basis, i, b);
-  to_float(b, bf, 4);
+  double o[3][4];
+  basis_at_point(basis, i, 0, o);
+  copy4f(bf, o[0]);
 }
 int get_NUBasis_dfuncs_s(NUBasis *basis, double x, float bf[4], float dbf[4]) {
-  double b[4], d[4];
-  const int i = get_NUBasis_dfuncs_d(basis, x, b, d);
-  to_float(b, bf, 4); to_float(d, dbf, 4);
+  double o[3][4];
+  const int i = basis_at(basis, x, 1, o);
+  copy4f(bf, o[0]); copy4f(dbf, o[1]);
   return i;
 }
 void get_NUBasis_dfuncs_si(NUBasis *basis, int i, float bf[4], float dbf[4]) {
-  double b[4], d[4];
-  get_NUBasis_dfuncs_di(basis, i, b, d);
-  to_float(b, bf, 4); to_float(d, dbf, 4);
+  double o[3][4];
+  basis_at_point(basis, i, 1, o);
+  copy4f(bf, o[0]); copy4f(dbf, o[1]);
 }
 int get_NUBasis_d2funcs_s(NUBasis *basis, double x, float bf[4], float dbf[4], float d2bf[4]) {
-  double b[4], d[4], e[4];
-  const int i = get_NUBasis_d2funcs_d(basis, x, b, d, e);
-  to_float(b, bf, 4); to_float(d, dbf, 4); to_float(e, d2bf, 4);
+  double o[3][4];
+  const int i = basis_at(basis, x, 2, o);
+  copy4f(bf, o[0]); copy4f(dbf, o[1]); copy4f(d2bf, o[2]);
   return i;
 }
 void get_NUBasis_d2funcs_si(NUBasis *basis, int i, float bf[4], float dbf[4], float d2bf[4]) {
-  double b[4], d[4], e[4];
-  get_NUBasis_d2funcs_di(basis, i, b, d, e);
-  to_float(b, bf, 4); to_float(d, dbf, 4); to_float(e, d2bf, 4);
+  double o[3][4];
+  basis_at_point(basis, i, 2, o);
+  copy4f(bf, o[0]); copy4f(dbf, o[1]); copy4f(d2bf, o[2]);
 }
 
 /* ------------------------------------------------------------------------ */
-/* 1-D solves (one real orbital: data[i*ds], coefficients p[i*ps])          */
+/* interpolation: the collocation operator, factored once per set call      */
 
-/* Rows: the left condition, the M data rows, the right condition; each row
- * holds three band entries and the right-hand side (bands[4r .. 4r+3]). */
-static int nub_solve_deriv(NUBasis *basis, const double *data, int ds, double *p, int ps,
-                           const double left[4], const double right[4]) {
-  const int M = basis->grid->num_points, N = M + 2;
-  double *bd = malloc(sizeof(double) * 4 * (size_t)N);
-  if (!bd) return -1;
-  for (int k = 0; k < 4; ++k) {
-    bd[k] = left[k];
-    bd[4 * (N - 1) + k] = right[k];
+/* Interpolation at the grid points of a basis with boundary rows (end
+ * derivatives) or a periodic wrap.  Unknowns: the M + 2 coefficients
+ * c_0 .. c_{M+1} (derivative conditions, M = num_points) or the M =
+ * num_points - 1 distinct ones c_1 .. c_M (periodic).
+ *   Derivative conditions: the boundary rows L (columns 0..2) and R
+ *   (columns M-1..M+1) are pivot rows for the outer columns: data row 0
+ *   minus a multiple of L loses column 0, data row M-1 minus a multiple of
+ *   R loses column M+1, leaving a tridiagonal system in c_1 .. c_M; c_0 and
+ *   c_{M+1} follow from L and R.  (L and R never have a zero outer entry:
+ *   the outermost function of an end interval has non-zero first and second
+ *   derivatives at the grid's end point.)
+ *   Periodic: a cyclic tridiagonal system, solved by bordering its last
+ *   unknown.
+ * The tridiagonal part is kept as Thomas factors: multiplier lo[r], inverse
+ * pivot iv[r], super entry up[r]. */
+typedef struct {
+  int periodic;
+  int M, n;            /* data points; unknowns of the tridiagonal part */
+  double *lo, *iv, *up;
+  /* derivative conditions: the boundary rows and the multiples of them
+   * taken from data rows 0 and M-1 */
+  double L[3], R[3], ml, mr;
+  /* periodic bordering: w = T^-1 p, the last row's entries q0 (col 0) and
+   * qe (col n-1), 1 / Schur complement */
+  double *w, q0, qe, zinv;
+} nub_op;
+
+static void op_free(nub_op *op) {
+  free(op->lo);
+  free(op->iv);
+  free(op->up);
+  free(op->w);
+}
+
+/* Thomas factors of the tridiagonal rows (sub, dia, sup) of length n, in place:
+ * sub becomes the multipliers, dia the inverse pivots. */
+static void thomas_factor(int n, double *sub, double *dia, const double *sup) {
+  dia[0] = 1.0 / dia[0];
+  for (int r = 1; r < n; ++r) {
+    sub[r] *= dia[r - 1];
+    dia[r] = 1.0 / (dia[r] - sub[r] * sup[r - 1]);
   }
-  for (int i = 0; i < M; ++i) {
-    get_NUBasis_funcs_di(basis, i, bd + 4 * (i + 1));
-    bd[4 * (i + 1) + 3] = data[(long)ds * i];
-  }
-  /* rows 0 and 1: the left condition spans three coefficients */
-  bd[1] /= bd[0];
-  bd[2] /= bd[0];
-  bd[3] /= bd[0];
-  bd[0] = 1.0;
-  bd[4 + 1] -= bd[4 + 0] * bd[1];
-  bd[4 + 2] -= bd[4 + 0] * bd[2];
-  bd[4 + 3] -= bd[4 + 0] * bd[3];
-  bd[0] = 0.0;
-  bd[4 + 2] /= bd[4 + 1];
-  bd[4 + 3] /= bd[4 + 1];
-  bd[4 + 1] = 1.0;
-  for (int r = 2; r < N - 1; ++r) {
-    double *row = bd + 4 * r;
-    const double *up = row - 4;
-    row[1] -= row[0] * up[2];
-    row[3] -= row[0] * up[3];
-    row[2] /= row[1];
-    row[3] /= row[1];
-    row[0] = 0.0;
-    row[1] = 1.0;
-  }
-  /* last row: the right condition spans the last three coefficients */
-  double *last = bd + 4 * (M + 1);
-  last[1] -= last[0] * bd[4 * (M - 1) + 2];
-  last[3] -= last[0] * bd[4 * (M - 1) + 3];
-  last[2] -= last[1] * bd[4 * M + 2];
-  last[3] -= last[1] * bd[4 * M + 3];
-  last[3] /= last[2];
-  last[2] = 1.0;
-  p[(long)ps * (M + 1)] = last[3];
-  for (int r = M; r > 0; --r) p[(long)ps * r] = bd[4 * r + 3] - bd[4 * r + 2] * p[(long)ps * (r + 1)];
-  p[0] = bd[3] - bd[1] * p[ps] - bd[2] * p[(long)ps * 2];
-  free(bd);
+}
+
+/* Solve in place for x[r * xs], r < n. */
+static void thomas_apply(int n, const double *lo, const double *iv, const double *up, double *x, long xs) {
+  for (int r = 1; r < n; ++r) x[r * xs] -= lo[r] * x[(r - 1) * xs];
+  x[(n - 1) * xs] *= iv[n - 1];
+  for (int r = n - 2; r >= 0; --r) x[r * xs] = (x[r * xs] - up[r] * x[(r + 1) * xs]) * iv[r];
+}
+
+/* The boundary row of a derivative condition at grid point `pt` (interval
+ * `pt`): the order-1 (DERIV1, FLAT) or order-2 (DERIV2, NATURAL) basis
+ * derivatives; -1 for a code that is not a derivative condition. */
+static int deriv_row(const NUBasis *basis, int code, int pt, double row[3]) {
+  int order;
+  if (code == DERIV1 || code == FLAT) order = 1;
+  else if (code == DERIV2 || code == NATURAL) order = 2;
+  else return -1;
+  double o[3][4];
+  basis_at_point(basis, pt, order, o);
+  for (int k = 0; k < 3; ++k) row[k] = o[order][k];
   return 0;
 }
 
-/* Cyclic system of the M = num_points - 1 distinct points: eliminated down
- * the band while the wrap entries collect in a last column. */
-static int nub_solve_periodic(NUBasis *basis, const double *data, int ds, double *p, int ps) {
-  const int M = basis->grid->num_points - 1;
-  double *bd = malloc(sizeof(double) * 4 * (size_t)M);
-  double *lc = malloc(sizeof(double) * (size_t)M);
-  if (!bd || !lc) { free(bd); free(lc); return -1; }
-  for (int i = 0; i < M; ++i) {
-    get_NUBasis_funcs_di(basis, i, bd + 4 * i);
-    bd[4 * i + 3] = data[(long)i * ds];
+static int op_build(nub_op *op, const NUBasis *basis, int lcode, int rcode) {
+  memset(op, 0, sizeof *op);
+  const int P = basis->grid->num_points;
+  op->periodic = lcode == PERIODIC;
+  op->M = op->periodic ? P - 1 : P;
+  const int M = op->M;
+  op->n = op->periodic ? M - 1 : M;
+  const int n = op->n;
+  op->lo = calloc((size_t)n + 1, sizeof(double));
+  op->iv = calloc((size_t)n + 1, sizeof(double));
+  op->up = calloc((size_t)n + 1, sizeof(double));
+  if (op->periodic) op->w = calloc((size_t)n + 1, sizeof(double));
+  if (!op->lo || !op->iv || !op->up || (op->periodic && !op->w)) { op_free(op); return -1; }
+  double o[3][4];
+  if (!op->periodic) {
+    if (deriv_row(basis, lcode, 0, op->L) || deriv_row(basis, rcode, M - 1, op->R)) { op_free(op); return -2; }
+    /* core row r (unknown c_{r+1}) is data row r: b0 c_r + b1 c_{r+1} + b2 c_{r+2} */
+    for (int r = 0; r < M; ++r) {
+      basis_at_point(basis, r, 0, o);
+      op->lo[r] = o[0][0];
+      op->iv[r] = o[0][1];
+      op->up[r] = o[0][2];
+    }
+    /* data row 0 minus ml L has no c_0; data row M-1 minus mr R no c_{M+1} */
+    op->ml = op->lo[0] / op->L[0];
+    op->iv[0] -= op->ml * op->L[1];
+    op->up[0] -= op->ml * op->L[2];
+    op->lo[0] = 0.0;
+    op->mr = op->up[M - 1] / op->R[2];
+    op->lo[M - 1] -= op->mr * op->R[0];
+    op->iv[M - 1] -= op->mr * op->R[1];
+    op->up[M - 1] = 0.0;
+    thomas_factor(n, op->lo, op->iv, op->up);
+    return 0;
   }
-  double *lr = bd + 4 * (M - 1);         /* the last row */
-  bd[2] /= bd[1];
-  bd[0] /= bd[1];
-  bd[3] /= bd[1];
-  bd[1] = 1.0;
-  lr[1] -= lr[2] * bd[0];
-  lr[3] -= lr[2] * bd[3];
-  lr[2] = -lr[2] * bd[2];
-  lc[0] = bd[0];
-  for (int r = 1; r < M - 1; ++r) {
-    double *row = bd + 4 * r;
-    const double *up = row - 4;
-    row[1] -= row[0] * up[2];
-    row[3] -= row[0] * up[3];
-    lc[r] = -row[0] * lc[r - 1];
-    row[0] = 0.0;
-    row[2] /= row[1];
-    row[3] /= row[1];
-    lc[r] /= row[1];
-    row[1] = 1.0;
-    if (r < M - 2) {
-      lr[3] -= lr[2] * row[3];
-      lr[1] -= lr[2] * lc[r];
-      lr[2] = -lr[2] * row[2];
+  /* periodic: row j is b0 u_{j-1} + b1 u_j + b2 u_{j+1} (indices mod M) */
+  double sub_last = 0.0, dia_last = 1.0, sup_last = 0.0;
+  for (int j = 0; j < M; ++j) {
+    basis_at_point(basis, j, 0, o);
+    if (j < n) {
+      op->lo[j] = o[0][0];
+      op->iv[j] = o[0][1];
+      op->up[j] = o[0][2];
+    } else {
+      sub_last = o[0][0];
+      dia_last = o[0][1];
+      sup_last = o[0][2];
     }
   }
-  /* the last row's [2] and [0] entries now sit in the same column */
-  lr[0] += lr[2];
-  lr[1] -= lr[0] * (bd[4 * (M - 2) + 2] + lc[M - 2]);
-  lr[3] -= lr[0] * bd[4 * (M - 2) + 3];
-  lr[3] /= lr[1];
-  p[(long)ps * M] = lr[3];
-  for (int r = M - 2; r >= 0; --r)
-    p[(long)ps * (r + 1)] = bd[4 * r + 3] - bd[4 * r + 2] * p[(long)ps * (r + 2)] - lc[r] * p[(long)ps * M];
-  p[0] = p[(long)ps * M];
-  p[(long)ps * (M + 1)] = p[ps];
-  p[(long)ps * (M + 2)] = p[(long)ps * 2];
-  free(bd);
-  free(lc);
+  /* the last unknown's column in rows 0 .. n-1: row 0's wrap and row n-1's super entry */
+  op->w[0] = op->lo[0];
+  op->w[n - 1] += op->up[n - 1];
+  op->q0 = sup_last;            /* last row, column 0 (wrap) */
+  op->qe = sub_last;            /* last row, column n-1 */
+  op->lo[0] = 0.0;
+  op->up[n - 1] = 0.0;
+  thomas_factor(n, op->lo, op->iv, op->up);
+  thomas_apply(n, op->lo, op->iv, op->up, op->w, 1);
+  op->zinv = 1.0 / (dia_last - op->q0 * op->w[0] - op->qe * op->w[n - 1]);
   return 0;
 }
 
-static int nub_find_coefs(NUBasis *basis, BCtype_d bc, const double *data, int ds, double *coefs, int cs) {
-  if (bc.lCode == PERIODIC) return nub_solve_periodic(basis, data, ds, coefs, cs);
-  const int M = basis->grid->num_points;
-  double bf[4], dbf[4], left[4], right[4];
-  if (bc.lCode == FLAT || bc.lCode == NATURAL) bc.lVal = 0.0;
-  if (bc.rCode == FLAT || bc.rCode == NATURAL) bc.rVal = 0.0;
-  if (bc.lCode == FLAT || bc.lCode == DERIV1) get_NUBasis_dfuncs_di(basis, 0, bf, left);
-  else if (bc.lCode == NATURAL || bc.lCode == DERIV2) get_NUBasis_d2funcs_di(basis, 0, bf, dbf, left);
-  else return -2;
-  left[3] = bc.lVal;
-  if (bc.rCode == FLAT || bc.rCode == DERIV1) get_NUBasis_dfuncs_di(basis, M - 1, bf, right);
-  else if (bc.rCode == NATURAL || bc.rCode == DERIV2) get_NUBasis_d2funcs_di(basis, M - 1, bf, dbf, right);
-  else return -2;
-  right[3] = bc.rVal;
-  return nub_solve_deriv(basis, data, ds, coefs, cs, left, right);
+/* Coefficients p[r * ps] (num_points + 2 of them) of one real component:
+ * data[j * ds] at the grid points, end values lval / rval. */
+static void op_apply(const nub_op *op, const double *data, long ds, double lval, double rval, double *p,
+                     long ps) {
+  const int M = op->M, n = op->n;
+  if (!op->periodic) {
+    double *u = p + ps;             /* c_1 .. c_M */
+    for (int j = 0; j < M; ++j) u[j * ps] = data[j * ds];
+    u[0] -= op->ml * lval;
+    u[(long)(M - 1) * ps] -= op->mr * rval;
+    thomas_apply(n, op->lo, op->iv, op->up, u, ps);
+    p[0] = (lval - op->L[1] * p[ps] - op->L[2] * p[2 * ps]) / op->L[0];
+    p[(long)(M + 1) * ps] =
+        (rval - op->R[0] * p[(long)(M - 1) * ps] - op->R[1] * p[(long)M * ps]) / op->R[2];
+    return;
+  }
+  /* u_r = c_{r+1}: solve the bordered system on p[ps ..] */
+  double *u = p + ps;
+  for (int j = 0; j < n; ++j) u[j * ps] = data[j * ds];
+  thomas_apply(n, op->lo, op->iv, op->up, u, ps);
+  const double last = (data[(long)(M - 1) * ds] - op->q0 * u[0] - op->qe * u[(n - 1) * ps]) * op->zinv;
+  for (int j = 0; j < n; ++j) u[j * ps] -= op->w[j] * last;
+  u[(long)(M - 1) * ps] = last;
+  /* the wrapped copies: c_0 = c_M, c_{M+1} = c_1, c_{M+2} = c_2 */
+  p[0] = p[(long)M * ps];
+  p[(long)(M + 1) * ps] = p[ps];
+  p[(long)(M + 2) * ps] = p[2 * ps];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -480,7 +482,10 @@ static void mark_dirty(einspline_nub1d *h) {
   pthread_mutex_unlock(&h->lock);
 }
 
-static void set_real(void *spline, int n, const double *data, int ds, BCtype_d bc, int comp, const char *fn) {
+/* Orbital n from `ncomp`-component data (interleaved): one factorisation,
+ * applied to each real component with its own end values. */
+static void set_orbital(void *spline, int n, const double *data, int lcode, int rcode, const double *lval,
+                        const double *rval, const char *fn) {
   einspline_nub1d *h = einspline_nub_of(spline);
   if (!h) { fprintf(stderr, "%s: spline was not created by this library\n", fn); return; }
   const int ncomp = h->ncomp;
@@ -489,28 +494,42 @@ static void set_real(void *spline, int n, const double *data, int ds, BCtype_d b
   double *base = ncomp == 2 ? (double *)h->pub.z.coefs : h->pub.d.coefs;
   const intptr_t xs = ncomp == 2 ? h->pub.z.x_stride : h->pub.d.x_stride;
   if (n < 0 || n >= nspl) { fprintf(stderr, "%s: spline index out of range\n", fn); return; }
-  double *coefs = base + (long)n * ncomp + comp;
-  const int rc = nub_find_coefs(basis, bc, data, ds, coefs, (int)(xs * ncomp));
+  if ((lcode == PERIODIC) != (rcode == PERIODIC) || (lcode == PERIODIC) != basis->periodic) {
+    fprintf(stderr, "%s: boundary conditions do not match the spline's periodicity\n", fn);
+    return;
+  }
+  nub_op op;
+  const int rc = op_build(&op, basis, lcode, rcode);
   if (rc == -1) { fprintf(stderr, "Out of memory in %s.\n", fn); abort(); }
   if (rc == -2) { fprintf(stderr, "%s: unsupported boundary condition\n", fn); return; }
+  for (int c = 0; c < ncomp; ++c) {
+    /* FLAT and NATURAL fix a zero end derivative */
+    const double lv = (lcode == FLAT || lcode == NATURAL) ? 0.0 : lval[c];
+    const double rv = (rcode == FLAT || rcode == NATURAL) ? 0.0 : rval[c];
+    op_apply(&op, data + c, ncomp, lv, rv, base + (long)n * ncomp + c, (long)xs * ncomp);
+  }
+  op_free(&op);
   mark_dirty(h);
+}
+
+static void set_real(multi_NUBspline_1d_d *spline, int num, const double *data, BCtype_d bc, const char *fn) {
+  if (!spline || !data) return;
+  set_orbital(spline, num, data, bc.lCode, bc.rCode, &bc.lVal, &bc.rVal, fn);
 }
 
 void set_multi_NUBspline_1d_d(multi_NUBspline_1d_d *spline, int num, double *data) {
   if (!spline) return;
-  set_real(spline, num, data, 1, spline->xBC, 0, "set_multi_NUBspline_1d_d");
+  set_real(spline, num, data, spline->xBC, "set_multi_NUBspline_1d_d");
 }
 
 void set_multi_NUBspline_1d_d_BC(multi_NUBspline_1d_d *spline, int num, double *data, BCtype_d xBC) {
-  set_real(spline, num, data, 1, xBC, 0, "set_multi_NUBspline_1d_d_BC");
+  set_real(spline, num, data, xBC, "set_multi_NUBspline_1d_d_BC");
 }
 
 static void set_cplx(multi_NUBspline_1d_z *spline, int num, complex_double *data, BCtype_z bc, const char *fn) {
-  if (!spline) return;
-  BCtype_d re = {bc.lCode, bc.rCode, bc.lVal_r, bc.rVal_r};
-  BCtype_d im = {bc.lCode, bc.rCode, bc.lVal_i, bc.rVal_i};
-  set_real(spline, num, (const double *)data, 2, re, 0, fn);
-  set_real(spline, num, (const double *)data + 1, 2, im, 1, fn);
+  if (!spline || !data) return;
+  const double lv[2] = {bc.lVal_r, bc.lVal_i}, rv[2] = {bc.rVal_r, bc.rVal_i};
+  set_orbital(spline, num, (const double *)data, bc.lCode, bc.rCode, lv, rv, fn);
 }
 
 void set_multi_NUBspline_1d_z(multi_NUBspline_1d_z *spline, int num, complex_double *data) {

@@ -1,11 +1,11 @@
 """1-D complex-double non-uniform multi-splines on the GPU
 (eval_multi_NUBspline_1d_z*, einspline_eval_nub_1d_z_batch) against the
 oracle (oracle/nub.py, pinned in test_nub_oracle.py) on the library's own
-coefficients.  The device arithmetic is the host's in the same order, so on
-general grids the results are expected bit for bit; center and log grids use
-the device log / log1p in their reverse maps, so a point within rounding of a
-grid point may take the neighbouring interval (the spline is C2 there), hence
-the 1e-12 bar."""
+coefficients.  The device evaluates the basis with the host library's
+triangular scheme (not the oracle's recursion), so results agree to
+rounding: 1e-12 normwise per point.  At a grid point the two may pick
+neighbouring intervals only where the spline is C2, which the same bar
+covers."""
 import numpy as np
 import pytest
 
@@ -63,14 +63,15 @@ def test_batch_matches_oracle(gkind, bc, N, rng):
                                                       mode, None), "nub batch")
         ea.synchronize()
         got = [b.download(np.complex128, (P, N)) for b in bufs[:mode + 1]]
-        for p, x in enumerate(xs):
-            # the oracle on the library's own reverse map, so the interval is the device's on general grids
-            ref = o.eval(name, x)
-            for a, r in zip(got, ref):
-                if gkind == "general":
-                    np.testing.assert_array_equal(a[p], r)
-                else:
-                    assert rel(a[p], r) < TOL * 1e3 or np.abs(a[p] - r).max() < TOL * 1e3
+        refs = [o.eval(name, x) for x in xs]
+        for q in range(mode + 1):
+            ref = np.array([r[q] for r in refs])
+            # per point, against that point's largest output (or, where the
+            # quantity vanishes at a point, e.g. a natural end's second
+            # derivative, the largest over all points)
+            scale = np.maximum(np.abs(ref).max(1), 1e-3 * np.abs(ref).max())
+            err = np.abs(got[q] - ref).max(1) / scale
+            assert err.max() < TOL, (gkind, name, float(err.max()), float(xs[err.argmax()]))
     ea.lib.destroy_multi_NUBspline_1d_hip(tab)
     s.destroy()
     g.destroy()
@@ -91,7 +92,7 @@ def test_single_point_api_and_refresh(rng):
             got = s.eval(mode, x)
             ref = o.eval("vgl" if mode == "vgh" else mode, x)
             for a, r in zip(got, ref):
-                np.testing.assert_array_equal(a, r)
+                assert rel(a, r) < TOL
     # interpolation at the nodes through the GPU, and the mirror follows set_*
     for k in (0, M // 2, M - 1):
         np.testing.assert_allclose(s.eval("v", g.points[k])[0], data[:, k], atol=1e-12)

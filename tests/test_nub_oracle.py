@@ -1,6 +1,8 @@
 """Non-uniform 1-D splines on the CPU: the oracle (oracle/nub.py) pinned by
 what the mathematics fixes, and the library's host code (grids, basis,
-set_multi_NUBspline_1d_*) checked against the oracle bit for bit.
+set_multi_NUBspline_1d_*), which computes them its own way (de Boor's
+triangular scheme, a factored collocation operator), checked against the
+oracle to rounding (1e-14 relative for the basis, 1e-13 for coefficients).
 
 Pins of the oracle:
   * on a uniform grid the non-uniform basis is the uniform cubic B-spline
@@ -144,26 +146,32 @@ def lib_grid(kind):
 def test_library_grids_and_basis_match_the_oracle(kind):
     lg, og = lib_grid(kind)
     np.testing.assert_allclose(lg.points, og.points, rtol=1e-15, atol=0)
-    og = nub.general_grid(lg.points)  # same points from here on: compare the arithmetic bit for bit
-    b = ea.lib.create_NUBasis(lg.ptr, False)
-    ob = nub.Basis(og, False)
+    og = nub.general_grid(lg.points)  # same points from here on
     q = lg.points
-    for x in np.random.default_rng(4).uniform(q[0], q[-1], 50):
-        a, da, d2a = (np.zeros(4) for _ in range(3))
-        i = ea.lib.get_NUBasis_d2funcs_d(b, float(x), a.ctypes.data, da.ctypes.data, d2a.ctypes.data)
-        assert i == lg.reverse(x)
-        oa, oda, od2a = ob.funcs(i, float(x))
-        np.testing.assert_array_equal(a, oa)
-        np.testing.assert_array_equal(da, oda)
-        np.testing.assert_array_equal(d2a, od2a)
-    ea.lib.destroy_NUBasis(b)
+    for periodic in (False, True):
+        b = ea.lib.create_NUBasis(lg.ptr, periodic)
+        ob = nub.Basis(og, periodic)
+        # the documented knots and inverse-span table
+        bs = b.contents
+        np.testing.assert_array_equal(np.ctypeslib.as_array(bs.xVals, (len(q) + 5,)), ob.xv)
+        np.testing.assert_allclose(np.ctypeslib.as_array(bs.dxInv, (3 * (len(q) + 2),)), ob.dx, rtol=1e-15)
+        xs = np.concatenate([np.random.default_rng(4).uniform(q[0], q[-1], 50), q, [q[0] - 0.1, q[-1] + 0.1]])
+        for x in xs:
+            a, da, d2a = (np.zeros(4) for _ in range(3))
+            i = ea.lib.get_NUBasis_d2funcs_d(b, float(x), a.ctypes.data, da.ctypes.data, d2a.ctypes.data)
+            # the interval brackets x (first and last clamp), as the oracle's does
+            assert i == og.reverse(x)
+            oa, oda, od2a = ob.funcs(i, float(x))
+            for got, ref in ((a, oa), (da, oda), (d2a, od2a)):
+                np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()))
+        ea.lib.destroy_NUBasis(b)
     lg.destroy()
 
 
 @pytest.mark.parametrize("bc", [(ea.PERIODIC, ea.PERIODIC), (ea.NATURAL, ea.FLAT),
                                 (ea.DERIV1, ea.DERIV2, 0.3 - 0.2j, -1.1 + 0.5j)])
 @pytest.mark.parametrize("kind", ["d", "z"])
-def test_library_set_matches_the_oracle_bitwise(kind, bc):
+def test_library_set_matches_the_oracle(kind, bc):
     pts = random_grid(16, np.random.default_rng(8))
     lg = ea.NUGrid.general(pts)
     bcv = bc if kind == "z" or len(bc) == 2 else (bc[0], bc[1], bc[2].real, bc[3].real)
@@ -175,8 +183,31 @@ def test_library_set_matches_the_oracle_bitwise(kind, bc):
         d = r.uniform(-1, 1, M) + (1j * r.uniform(-1, 1, M) if kind == "z" else 0)
         s.set(n, d)
         o.set(n, d)
-    np.testing.assert_array_equal(s.coefs(), o.coefs)
+    c = s.coefs()
+    assert np.abs(c - o.coefs).max() <= 1e-13 * np.abs(o.coefs).max()
     s.destroy()
+    lg.destroy()
+
+
+@pytest.mark.parametrize("bc", [(ea.NATURAL, ea.NATURAL), (ea.DERIV2, ea.DERIV2, 0.7, -0.3),
+                                (ea.FLAT, ea.FLAT), (ea.DERIV1, ea.DERIV2, 2.0, 1.0)])
+def test_uniform_grid_set_is_the_uniform_solve(bc):
+    """On a uniform grid the non-uniform spline is the uniform one: the
+    library's 1-D uniform set (create_multi_UBspline_1d_d, its own solver)
+    gives the same coefficients for every end condition, including the
+    second-derivative ones whose first collocation row would otherwise meet
+    a zero pivot."""
+    n = 11
+    pts = np.linspace(0.0, 2.0, n)
+    lg = ea.NUGrid.general(pts)
+    s = ea.NUBSpline1D("d", lg, bc, 1)
+    u = ea.HostSpline("d", (n,), (0.0,), (2.0,), 1, bc=[bc])
+    d = np.random.default_rng(3).uniform(-1, 1, n)
+    s.set(0, d)
+    u.set(0, d)
+    np.testing.assert_allclose(s.coefs()[:, 0], u.coefs()[:, 0], rtol=0, atol=1e-13)
+    s.destroy()
+    u.destroy()
     lg.destroy()
 
 
