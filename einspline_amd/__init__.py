@@ -58,7 +58,7 @@ MULTI_U1D, MULTI_U2D, MULTI_U3D = 6, 7, 8
 STORE_F32, STORE_F64, STORE_BF16, STORE_F16 = range(4)
 MODES = ("v", "vg", "vgl", "vgh")          # autotuner modes 0..3; "vghgh" is batch-only
 NOALLOC_MAX_POINTS = 1 << 20               # EINSPLINE_NOALLOC_MAX_POINTS
-ORDER_GIVEN, ORDER_LOCALITY = 0, 1
+ORDER_GIVEN, ORDER_LOCALITY, ORDER_SLABS = 0, 1, 2   # ORDER_SLABS is a flag added to either order
 KERNEL_VECTOR, KERNEL_MATRIX = 0, 1
 
 E_OK, E_INVAL, E_NOMEM, E_HIP, E_RANGE, E_BUDGET, E_BUSY = 0, -1, -2, -3, -4, -5, -6

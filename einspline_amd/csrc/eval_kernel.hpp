@@ -76,6 +76,8 @@ struct Params {
   int store;                 // result store policy: STORE_STAGED_NT, STORE_STAGED, STORE_DIRECT
   const int *perm;           // optional traversal order: slot s evaluates point perm[s]
   int xcd_runs;              // 1: each XCD takes one contiguous run of blocks
+  int slabs;                 // > 1: orbital slabs outermost (a wavefront per point and slab,
+                             // slab-major); the number of slabs per point
 };
 
 // Result store policies.  STAGED: gradient/Hessian blocks of a wavefront go
@@ -297,6 +299,43 @@ __device__ __forceinline__ unsigned logical_block(unsigned b, unsigned nb, int r
   if (!runs) return b;
   const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
+// Orbital slabs outermost: the point and slab of wavefront `wib` of block
+// `blk` (P.slabs slabs per point, `wpb` wavefronts per block).  In the
+// given order the waves run slab-major over all points.  With XCD runs
+// (the locality order) XCD x = blk % 8 owns the x-th contiguous eighth of
+// the sorted points and walks it slab-major, so each XCD's L2 holds one
+// slab of its own region's rows at a time; the grid then has 8 x
+// ceil(ceil(npoints / 8) x slabs / wpb) blocks (slab_blocks).  False for a
+// wavefront with no work.
+__device__ __forceinline__ bool slab_wave(const Params &P, unsigned blk, unsigned wpb, unsigned wib, int &p,
+                                          int &slab) {
+  unsigned pt, sl;
+  if (P.xcd_runs) {
+    const unsigned q = ((unsigned)P.npoints + 7u) >> 3;        // points per XCD
+    const unsigned lw = (blk >> 3) * wpb + wib;                // wave within the XCD
+    sl = lw / q;
+    pt = (blk & 7u) * q + (lw - sl * q);
+  } else {
+    const unsigned w = blk * wpb + wib;
+    sl = w / (unsigned)P.npoints;
+    pt = w - sl * (unsigned)P.npoints;
+  }
+  p = uniform((int)pt);
+  slab = uniform((int)sl);
+  return sl < (unsigned)P.slabs && pt < (unsigned)P.npoints;
+}
+
+// Blocks of `wpb` wavefronts for `waves_per_point` x npoints wavefronts in
+// the slab-major mapping of slab_wave.
+inline unsigned slab_blocks(const Params &P, int wpb) {
+  const long s = P.slabs > 1 ? P.slabs : 1;
+  if (P.slabs > 1 && P.xcd_runs) {
+    const long q = ((long)P.npoints + 7) / 8;
+    return (unsigned)(8 * ((q * s + wpb - 1) / wpb));
+  }
+  return (unsigned)(((long)P.npoints * s + wpb - 1) / wpb);
 }
 
 // Make LDS writes of this wavefront visible to its other lanes.
@@ -863,10 +902,19 @@ __global__ void __launch_bounds__(256, (Occupancy<S, LB, MODE, UNIF>::waves)) ev
   // npoints * chunks < 2^31 is checked on the host, so 32-bit math suffices
   const unsigned b = logical_block(blockIdx.x, gridDim.x, P.xcd_runs);
   const unsigned gid = b * blockDim.x + threadIdx.x;
-  int p = (int)(gid / (unsigned)P.chunks);
-  if (UNIF) p = uniform(p);
-  const int ch = (int)(gid - (unsigned)p * (unsigned)P.chunks);
-  if (p >= P.npoints) return;
+  int p, ch;
+  if (UNIF && P.slabs > 1) {
+    // orbital slabs outermost (the same 64-lane piece of the point's chunks
+    // per slab): slab_wave() maps the wavefront to its point and slab
+    int slab;
+    if (!slab_wave(P, blockIdx.x, blockDim.x >> 6, threadIdx.x >> 6, p, slab)) return;
+    ch = slab * 64 + (int)(threadIdx.x & 63u);
+  } else {
+    p = (int)(gid / (unsigned)P.chunks);
+    if (UNIF) p = uniform(p);
+    ch = (int)(gid - (unsigned)p * (unsigned)P.chunks);
+    if (p >= P.npoints) return;
+  }
   if (P.perm) {
     p = P.perm[p];
     if (UNIF) p = uniform(p);

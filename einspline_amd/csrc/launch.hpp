@@ -57,6 +57,7 @@ template <class S, int LB, int MODE, bool CPLX, bool UNIF>
 hipError_t go_api(const Launch &L, const Params &P, hipStream_t s) {
   const long total = (long)P.npoints * P.chunks;
   dim3 block(L.block), grid((unsigned)((total + L.block - 1) / L.block));
+  if (UNIF && P.slabs > 1) grid = dim3(slab_blocks(P, L.block / 64));
   const size_t lds = StageLds<S, LB, MODE, UNIF>::bytes(L.block, P.store);
   if (L.api == API_STRIDED) {
     if constexpr (MODE != MODE_QMC)

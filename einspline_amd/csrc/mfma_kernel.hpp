@@ -87,11 +87,16 @@ constexpr int kMfmaStage = 128 * (1 + 3 + 9);
 // The wavefront's point (blocks of four wavefronts, XCD runs and the
 // traversal order as in eval3d): its cell and basis values (value, first and
 // second derivative weights per axis).  False past the last point.
-__device__ __forceinline__ bool mfma_point(const Params &P, int &p, int &ix, int &iy, int &iz,
+__device__ __forceinline__ bool mfma_point(const Params &P, int &p, int &slab, int &ix, int &iy, int &iz,
                                            float (&wx)[3][4], float (&wy)[3][4], float (&wz)[3][4]) {
-  const unsigned b = logical_block(blockIdx.x, gridDim.x, P.xcd_runs);
-  p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-  if (p >= P.npoints) return false;
+  slab = -1;                          // all slabs of the point
+  if (P.slabs > 1) {                  // orbital slabs outermost: one slab per wavefront
+    if (!slab_wave(P, blockIdx.x, blockDim.x >> 6, threadIdx.x >> 6, p, slab)) return false;
+  } else {
+    const unsigned b = logical_block(blockIdx.x, gridDim.x, P.xcd_runs);
+    p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    if (p >= P.npoints) return false;
+  }
   if (P.perm) p = uniform(P.perm[p]);
   const double *pp = (const double *)P.pos + 3l * p;
   float tx, ty, tz;
@@ -156,9 +161,9 @@ eval3d_mfma(const Params P) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, r = lane & 15;
   float *const stage = stage_all + (threadIdx.x >> 6) * kMfmaStage;
-  int p, ix, iy, iz;
+  int p, slab, ix, iy, iz;
   float wx[3][4], wy[3][4], wz[3][4];
-  if (!mfma_point(P, p, ix, iy, iz, wx, wy, wz)) return;
+  if (!mfma_point(P, p, slab, ix, iy, iz, wx, wy, wz)) return;
 
   // FP16 holds magnitudes up to 65504 only: points outside the grid box
   // extrapolate with weights that grow like t^3 per axis, so products bounded
@@ -276,14 +281,17 @@ eval3d_mfma(const Params P) {
   // Software pipeline over half-slabs: the rows of the next half are in
   // flight while the current one is multiplied and the slab is stored.
   uint4 R0[8], R1[8];
-  load_half(R0, 0, 0);
-  for (int s0 = 0; s0 < end; s0 += 128) {
+  // all slabs of the point, or only the wavefront's slab (orbital slabs outermost)
+  const int s_first = slab < 0 ? 0 : slab * 128;
+  const int s_last = slab < 0 ? end : min(end, s_first + 128);
+  load_half(R0, s_first, 0);
+  for (int s0 = s_first; s0 < s_last; s0 += 128) {
     f4 acc[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
     load_half(R1, s0, 1);
     mma_half(R0, acc, 0);
-    if (s0 + 128 < end) load_half(R0, s0 + 128, 0);
+    if (s0 + 128 < s_last) load_half(R0, s0 + 128, 0);
     mma_half(R1, acc, 1);
     // stage the slab (orbitals past the valid ones are staged but not
     // copied out)
@@ -338,9 +346,9 @@ eval3d_mfma32(const Params P) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, r = lane & 15;
   float *const stage = stage_all + (threadIdx.x >> 6) * kMfma32Stage;
-  int p, ix, iy, iz;
+  int p, slab, ix, iy, iz;
   float wx[3][4], wy[3][4], wz[3][4];
-  if (!mfma_point(P, p, ix, iy, iz, wx, wy, wz)) return;
+  if (!mfma_point(P, p, slab, ix, iy, iz, wx, wy, wz)) return;
 
   // A operand of chunk kc: W[q = r][(i, j, g)] with i = kc / 4, j = kc % 4
   const int q = r;
@@ -397,14 +405,17 @@ eval3d_mfma32(const Params P) {
   };
 
   f4 R0[8], R1[8];
-  load_half(R0, 0, 0);
-  for (int s0 = 0; s0 < end; s0 += 64) {
+  // all slabs of the point, or only the wavefront's slab (orbital slabs outermost)
+  const int s_first = slab < 0 ? 0 : slab * 64;
+  const int s_last = slab < 0 ? end : min(end, s_first + 64);
+  load_half(R0, s_first, 0);
+  for (int s0 = s_first; s0 < s_last; s0 += 64) {
     f4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
     load_half(R1, s0, 1);
     mma_half(R0, acc, 0);
-    if (s0 + 64 < end) load_half(R0, s0 + 64, 0);
+    if (s0 + 64 < s_last) load_half(R0, s0 + 64, 0);
     mma_half(R1, acc, 1);
     typedef float v4 __attribute__((ext_vector_type(4)));
     if (g == 0) {

@@ -284,6 +284,14 @@ int einspline_set_store_policy(multi_UBspline_3d_hip *spline, int policy);
  * Applies to the 3-D batched calls: the strided einspline_eval_*_batch
  * functions and the pointer-array eval_multi_multi_UBspline_3d_*_hip ones. */
 enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
+/* Either order may add EINSPLINE_ORDER_SLABS (order | EINSPLINE_ORDER_SLABS):
+ * orbital slabs outermost.  When a point's row spans several wavefronts'
+ * widths (vector kernel: more than 64 lanes per point; matrix kernels: more
+ * than 128 orbitals, or 64 for FP32 tables), all points evaluate the first
+ * slab of orbitals, then the second, and so on, so the rows of one slab are
+ * what the caches hold at a time.  Results are identical.  Applies to the
+ * strided einspline_eval_*_batch calls; elsewhere it is ignored. */
+enum { EINSPLINE_ORDER_SLABS = 2 };
 int einspline_set_point_order(multi_UBspline_3d_hip *spline, int order);
 
 /* Which kernel evaluates vgh on a BF16, FP16 or FP32 table: the vector

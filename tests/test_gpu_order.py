@@ -131,7 +131,7 @@ def test_autotune_reports_an_order_and_keeps_results(rng):
     before = gpu_eval(tab, "vgh", pts)
     tab.autotune("vgh", 4096)
     lb, blk, pol, order = tab.tuning()
-    assert order in (ea.ORDER_GIVEN, ea.ORDER_LOCALITY)
+    assert order in (0, 1, 2, 3)          # given / locality, optionally with orbital slabs
     # the matrix-core kernel (a candidate for vgh) sums in another order
     for a, b in zip(gpu_eval(tab, "vgh", pts), before):
         if tab.kernel == ea.KERNEL_VECTOR:
@@ -153,7 +153,7 @@ def test_autotune_on_clustered_caller_points(rng):
     before = gpu_eval(tab, "vgh", pts)
     lb, blk, pol, ms = tab.autotune_points("vgh", pts)
     assert ms > 0.0 and (lb, blk, pol) == tab.tuning()[:3]
-    assert tab.tuning()[3] in (ea.ORDER_GIVEN, ea.ORDER_LOCALITY)
+    assert tab.tuning()[3] in (0, 1, 2, 3)
     for a, b in zip(gpu_eval(tab, "vgh", pts), before):
         if tab.kernel == ea.KERNEL_VECTOR:
             np.testing.assert_array_equal(a, b)
@@ -161,3 +161,43 @@ def test_autotune_on_clustered_caller_points(rng):
             assert normwise(a, b) < TOL["s"]
     with pytest.raises(ea.EinsplineError):
         tab.autotune_points("vgh", np.zeros((0, 3)))
+
+
+@pytest.mark.parametrize("storage,kernel,tile,N", [(ea.STORE_F32, ea.KERNEL_VECTOR, (8, 256, 2), 384),
+                                                   (ea.STORE_F32, ea.KERNEL_VECTOR, (16, 128, 0), 512),
+                                                   (ea.STORE_BF16, ea.KERNEL_VECTOR, (8, 64, 1), 512),
+                                                   (ea.STORE_BF16, ea.KERNEL_MATRIX, (16, 256, 0), 300),
+                                                   (ea.STORE_F16, ea.KERNEL_MATRIX, (16, 256, 1), 300),
+                                                   (ea.STORE_F32, ea.KERNEL_MATRIX, (16, 256, 1), 300)])
+def test_orbital_slab_orders_are_bitwise_identical(storage, kernel, tile, N, rng):
+    """EINSPLINE_ORDER_SLABS (orbital slabs outermost) with either point order
+    gives the bits of the plain given order.  Rows span 2-5 wavefront widths
+    (vector kernel: 128-192 lanes per point; matrix kernels: 300 orbitals,
+    the last slab partial); 5000 points, so the locality sort runs and the
+    per-XCD split of the points is uneven."""
+    P = 5000
+    h = ea.HostSpline("d", (11, 9, 12), (0, 0, 0), (1, 1, 1), N)
+    h.set_block(0, rng.uniform(-1, 1, (N, 11, 9, 12)))
+    tab = ea.DeviceTable(h, storage=storage)
+    tab.set_kernel(kernel)
+    tab.set_tile(tile[0], tile[1])
+    tab.set_store_policy(tile[2])
+    pts = points(rng, P)
+    tab.set_point_order(ea.ORDER_GIVEN)
+    base = gpu_eval(tab, "vgh", pts)
+    for order in (ea.ORDER_SLABS, ea.ORDER_LOCALITY | ea.ORDER_SLABS, ea.ORDER_LOCALITY):
+        tab.set_point_order(order)
+        assert tab.tuning()[3] == order
+        for a, b in zip(gpu_eval(tab, "vgh", pts), base):
+            np.testing.assert_array_equal(a, b)
+        # packed Hessians too
+        for a, b in zip(gpu_eval(tab, "vgh", pts, packed=True), gpu_eval_given_packed(tab, pts)):
+            np.testing.assert_array_equal(a, b)
+
+
+def gpu_eval_given_packed(tab, pts):
+    order = tab.tuning()[3]
+    tab.set_point_order(ea.ORDER_GIVEN)
+    out = gpu_eval(tab, "vgh", pts, packed=True)
+    tab.set_point_order(order)
+    return out
