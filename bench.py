@@ -20,7 +20,9 @@ Poisson load (all under "extra").
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -39,8 +41,10 @@ VALU_FP32_PEAK_TFLOPS = 78.6
 FLOPS_PER_EVAL = {"v": 2 * (4 * 16 + 16), "vg": 2 * (8 * 16 + 4 * 16), "vgl": 2 * (12 * 16 + 7 * 16),
                   "vgh": 704}
 SERVE_SLO_P99_S = 1e-3          # serving capacity is quoted at this p99 latency
+SERVE_SLO_SWAP_P99_S = 20e-3    # ... and, when tables must be swapped, also at this one
 METRIC = "orbital-vgh evals/sec + %HBM roofline, FP32 vs BF16 table; serving p99 latency"
 ORDER_NAMES = ("given", "locality")
+ORDER_LABELS = ("given", "locality", "given+slabs", "locality+slabs")
 
 
 KERNELS = ("vector", "matrix")
@@ -117,29 +121,74 @@ def bytes_per_eval(coef_bytes, mode, packed, out_bytes):
     return 64 * coef_bytes + nout * out_bytes
 
 
-def cpu_baseline(kind, host, pts, mode, target_s=10.0):
-    """The oracle (restatement of the reference's std eval) on the host cores,
-    on a bounded sample of the same workload."""
+def host_cpu():
+    """What the host offers the CPU baseline: the thread count it uses (the
+    CPUs this process may run on, capped by OMP_NUM_THREADS when set, which
+    is how the box allots its CPU share) and the facts to report beside it."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc", "--all"], capture_output=True, text=True, timeout=10).stdout)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = None
+    return threads, {"model": model, "nproc_all": nproc, "os_cpu_count": os.cpu_count(),
+                     "affinity_cpus": aff, "OMP_NUM_THREADS": omp}
+
+
+def native_oracle():
+    """The oracle's C source built for this host (-O3 -march=native -fopenmp)
+    into a temporary directory, or the prebuilt portable library if gcc
+    fails.  Returns (library or None, build description)."""
+    import oracle
+    src = os.path.join(ROOT, "oracle", "oracle.c")
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+    cmd = ["gcc", "-std=gnu11", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", src, "-o", out, "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=180)
+        return oracle.load_library(out), " ".join(cmd[:6]) + " (built on this host)"
+    except (OSError, subprocess.SubprocessError) as e:
+        log(f"[bench] native oracle build failed ({e}); timing the prebuilt portable build")
+        return None, "prebuilt oracle/liboracle.so (-O3 -march=x86-64-v2 -ffp-contract=off)"
+
+
+def cpu_baseline(kind, host, pts, mode, target_s=3.0):
+    """The oracle (restatement of the reference's std eval) on the host's
+    CPUs, one shared read-only table, OpenMP over points, on a bounded sample
+    of the same workload (about target_s x 3 seconds of CPU work)."""
     import oracle
 
     N = host.N
     M = host.M
-    o = oracle.Table(kind, M, (0, 0, 0), (1, 1, 1), N)
+    lib, build = native_oracle()
+    o = oracle.Table(kind, M, (0, 0, 0), (1, 1, 1), N, library=lib)
     o.coefs[...] = host.coefs().reshape(o.coefs.shape)
-    threads = min(os.cpu_count() or 1, 16)
+    threads, facts = host_cpu()
     n = 128
     t0 = time.perf_counter()
     o.eval_batch(mode, pts[:n], nthreads=threads)
     dt = time.perf_counter() - t0
-    n2 = int(min(len(pts), max(n, n * target_s / max(dt, 1e-6) / 3)))
-    best = None
-    for _ in range(3):
+    n2 = int(min(len(pts), max(n, n * target_s / max(dt, 1e-6))))
+    o.eval_batch(mode, pts[:min(n2, 1024)], nthreads=threads)          # warm-up
+    runs = []
+    for _ in range(5):
         t0 = time.perf_counter()
         o.eval_batch(mode, pts[:n2], nthreads=threads)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-    return {"value": n2 * N / best, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"{n2} of the {len(pts)} points x {N} orbitals, {mode}, best of 3 runs"}
+        runs.append(time.perf_counter() - t0)
+    med = float(np.median(runs))
+    return {"value": n2 * N / med, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n2} of the {len(pts)} points x {N} orbitals, {mode}, one shared table, "
+                      f"OpenMP over points, median of 5 runs after a warm-up",
+            "build": build, "host": facts}
 
 
 def load_traffic(name):
@@ -159,42 +208,103 @@ def load_traffic(name):
 # workloads
 
 
+# Candidates of the headline's given-order tuning: vector kernel lane bytes x
+# block x store policy, and the FP32 matrix-core kernel with either store
+# kind.  A candidate within 3% of the profiled tile keeps the profiled tile,
+# so that the committed counter summary describes the launch that is timed.
+HEADLINE_PROFILE = "eval3d_f32_vgh_48c_n256_p32768.given.traffic.json"
+
+
+def _time_config(tab, pos, P, outs, packed, reps=5, trials=3):
+    """Best of `trials` event-timed runs of `reps` launches (ms per launch)."""
+    stream = ea.Stream()
+    tab.eval_batch("vgh", pos, P, outs, packed, stream)
+    stream.synchronize()
+    e0, e1 = ea.Event(), ea.Event()
+    best = None
+    for _ in range(trials):
+        e0.record(stream)
+        for _ in range(reps):
+            tab.eval_batch("vgh", pos, P, outs, packed, stream)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        best = ms if best is None else min(best, ms)
+    return best
+
+
+def tune_given_order(tab, pos, P, outs, packed, profiled):
+    """Time every candidate in the given point order and apply the fastest
+    (or the profiled tile when it is within 3%).  Returns the timings."""
+    tab.set_point_order(ea.ORDER_GIVEN)
+    rows = []
+    for kern in (ea.KERNEL_VECTOR, ea.KERNEL_MATRIX):
+        for lb in ((8, 16, 32) if kern == ea.KERNEL_VECTOR else (16,)):
+            for blk in ((64, 128, 256) if kern == ea.KERNEL_VECTOR else (256,)):
+                for pol in ((0, 1, 2) if kern == ea.KERNEL_VECTOR else (0, 1)):
+                    tab.set_kernel(kern)
+                    tab.set_tile(lb, blk)
+                    tab.set_store_policy(pol)
+                    rows.append({"kernel": KERNELS[kern], "lane_bytes": lb, "block": blk, "store_policy": pol,
+                                 "ms": _time_config(tab, pos, P, outs, packed)})
+    best = min(rows, key=lambda r: r["ms"])
+    for r in sorted(rows, key=lambda r: r["ms"])[:6]:
+        log(f"[bench]   given-order candidate {r['kernel']} {r['lane_bytes']} B x {r['block']}, "
+            f"store policy {r['store_policy']}: {r['ms']:.4f} ms")
+    pick = best
+    if profiled:
+        for r in rows:
+            if (r["kernel"], r["lane_bytes"], r["block"], r["store_policy"]) == profiled and \
+                    r["ms"] <= 1.03 * best["ms"]:
+                pick = r
+    tab.set_kernel(KERNELS.index(pick["kernel"]))
+    tab.set_tile(pick["lane_bytes"], pick["block"])
+    tab.set_store_policy(pick["store_policy"])
+    return rows, best
+
+
 def headline(args):
+    """FP32 vgh, 48^3 x 256, 32768 uniform points in the order given (the
+    unsorted random order: every stencil row a point needs is requested in
+    that point's turn)."""
     M, N, P = 48, 256, 32768
     t0 = time.perf_counter()
     host = build_table("s", M, N, seed=2024)
     log(f"[bench] built 48^3 x 256 FP32 table in {time.perf_counter() - t0:.1f}s")
     tab = ea.DeviceTable(host)
-    if args.lane_bytes or args.block:
-        tab.set_tile(args.lane_bytes, args.block)
-        tab.set_store_policy(args.store_policy)
-        tab.set_point_order(max(args.point_order, 0))
-        tab.set_kernel(KERNELS.index(args.kernel) if args.kernel else ea.KERNEL_VECTOR)
-    else:
-        tab.autotune("vgh", P)
-        if args.point_order >= 0:
-            tab.set_point_order(args.point_order)
-    tile = tab.tuning()            # (lane bytes, block, store policy, point order)
     rng = np.random.default_rng(99)
     pts = rng.random((P, 3))
     pos = ea.DeviceBuffer.from_array(pts)
     packed = args.packed
     outs = tab.alloc_outputs("vgh", P, packed)
+    prof = load_traffic(HEADLINE_PROFILE)
+    profiled = None
+    if prof:
+        t = prof.get("tile", {})
+        profiled = (t.get("kernel", "vector"), t.get("lane_bytes"), t.get("block"), t.get("store_policy"))
+    sweep = None
+    if args.lane_bytes or args.block:
+        tab.set_kernel(KERNELS.index(args.kernel) if args.kernel else ea.KERNEL_VECTOR)
+        tab.set_tile(args.lane_bytes, args.block)
+        tab.set_store_policy(args.store_policy)
+        tab.set_point_order(max(args.point_order, 0))
+    else:
+        sweep, best = tune_given_order(tab, pos, P, outs, packed, profiled)
+        if args.point_order > 0:
+            tab.set_point_order(args.point_order)
+    tile = tab.tuning()            # (lane bytes, block, store policy, point order)
+    kern_name = KERNELS[tab.kernel]
     wall, kern = time_batches(tab, "vgh", pos, P, outs, args.steps, args.warmup, packed)
     evals = P * N
     bpe = bytes_per_eval(4, "vgh", packed, 4)
     achieved = evals * bpe / (kern * 1e-3) / 1e9
-    log(f"[bench] FP32 vgh: {wall:.4f} ms/step wall, {kern:.4f} ms/launch (events)")
-    # measured HBM traffic of the same call (rocprofv3 PMC, profiles/); used
-    # only when it was taken at this tile, store policy and point order
+    log(f"[bench] FP32 vgh ({ORDER_LABELS[tile[3]]} order, {kern_name} {tile[:3]}): {wall:.4f} ms/step wall, "
+        f"{kern:.4f} ms/launch (events)")
+    # HBM traffic of this launch, from the committed counter summary of the
+    # same tile and order (rocprofv3 cannot run inside this process)
     traffic = None
-    if tab.kernel == ea.KERNEL_VECTOR:
-        for suffix in ("", ".nt"):
-            t = load_traffic(f"eval3d_f32_vgh_48c_n256_p32768.{ORDER_NAMES[tile[3]]}{suffix}.traffic.json")
-            if t and (t.get("tile", {}).get("lane_bytes"), t.get("tile", {}).get("block"),
-                      t.get("tile", {}).get("store_policy")) == tuple(tile[:3]):
-                traffic = t
-                break
+    if prof and profiled == (kern_name, tile[0], tile[1], tile[2]) and tile[3] == ea.ORDER_GIVEN and not packed:
+        traffic = prof
     flops = evals * FLOPS_PER_EVAL["vgh"]
     res = {
         "metric": METRIC,
@@ -211,59 +321,95 @@ def headline(args):
         "data": "synthetic: 8 random plane waves per orbital (|k_i|<=4, N(0,1) amplitudes), "
                 "uniform random points, seeded",
         "config": {"workload": "multi_UBspline_3d_s periodic 48^3, 256 orbitals, vgh at 32768 "
-                               "points (256 walkers x 128 electrons), one launch",
+                               "points (256 walkers x 128 electrons) in the given (unsorted random) order, "
+                               "one launch",
                    "grid": [M, M, M], "orbitals": N, "points": P, "mode": "vgh",
                    "table": "f32", "hessian_layout": "packed6" if packed else "full9",
                    "table_bytes": int(tab.bytes), "tile_lane_bytes": tile[0],
                    "tile_block": tile[1], "store_policy": tile[2],
-                   "point_order": ORDER_NAMES[tile[3]], "kernel": KERNELS[tab.kernel]},
+                   "point_order": ORDER_LABELS[tile[3]], "kernel": kern_name},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "bytes_per_eval": bpe, "kernel_ms": kern,
                      "traffic_GBs": (traffic["bytes_per_launch"] / (kern * 1e-3) / 1e9) if traffic else None,
+                     "traffic_source": (f"profiles/{HEADLINE_PROFILE}: rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
+                                        f"tile in the given order, scaled by the copy-kernel calibration"
+                                        if traffic else None),
                      "flops_per_eval": FLOPS_PER_EVAL["vgh"],
                      "achieved_TFLOPs": flops / (kern * 1e-3) / 1e12,
                      "valu_fp32_peak_TFLOPs": VALU_FP32_PEAK_TFLOPS,
                      "frac_valu": flops / (kern * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS,
-                     "note": ("algorithmic bytes count every stencil row a point reads; the table "
-                              "(135.8 MB) fits the 256 MB Infinity Cache"
-                              + ("; in the locality order most rows are re-read from L2, so 'achieved' "
-                                 "exceeds the HBM peak: 'traffic' is what reaches HBM and the call is "
-                                 "bounded by FP32 VALU issue and load latency (frac_valu: against 78.6 "
-                                 "TFLOP/s, FP32 FMA without the packed forms); kernel_ms is the whole call: "
-                                 "two sort kernels + eval3d" if tile[3] == ea.ORDER_LOCALITY else ""))},
+                     "note": "achieved = algorithmic bytes (64 x 4 B stencil reads + 13 x 4 B results per "
+                             "(point, orbital)) / event-timed launch; the 135.8 MB table fits the 256 MB "
+                             "Infinity Cache, so 'traffic' counts bytes past L2, served in part by it"},
     }
+    if sweep:
+        res.setdefault("extra", {})["headline_given_order_candidates"] = {
+            "fastest": best, "rows": sweep,
+            "note": "each candidate: best of 3 event-timed runs of 5 launches in the given order; the "
+                    "profiled tile is kept when within 3% of the fastest"}
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, host cores)")
         res["cpu_baseline"] = cpu_baseline("s", host, pts, "vgh")
     # the drop-in single-point API: one launch + result copy per call
     if not args.no_extra:
-        x = pts[:200]
-        host.eval("vgh", *x[0])                 # builds the device mirror
+        x = [tuple(float(c) for c in p) for p in pts[:400]]
+        fn = ea.lib.eval_multi_UBspline_3d_s_vgh
+        vb, gb, hb = np.zeros(N, np.float32), np.zeros(3 * N, np.float32), np.zeros(9 * N, np.float32)
+        vp, gp, hp = vb.ctypes.data, gb.ctypes.data, hb.ctypes.data
+        fn(host.ptr, *x[0], vp, gp, hp)               # builds the device mirror
         t = []
         for p in x:
             t0 = time.perf_counter()
-            host.eval("vgh", *p)
+            fn(host.ptr, p[0], p[1], p[2], vp, gp, hp)
             t.append(time.perf_counter() - t0)
         res.setdefault("extra", {})["single_point_vgh_call_us"] = {
             "median": float(np.median(t) * 1e6), "p99": float(np.percentile(t, 99) * 1e6),
-            "note": "eval_multi_UBspline_3d_s_vgh through ctypes: H2D position, launch, D2H of 256 x 13 floats"}
-    # the other traversal order at the same tile, for the record
+            "note": "eval_multi_UBspline_3d_s_vgh (256 orbitals, 13 results each) called through ctypes "
+                    "into preallocated host arrays; one launch + stream synchronisation per call (an empty "
+                    "launch + synchronisation alone takes about 11 us on this GPU)"}
+    # the locality order: the library sorts the batch by cell first (two small
+    # kernels inside the timed call); most stencil rows then come from L2
     if not args.no_extra:
-        other = 1 - tile[3]
-        tab.set_point_order(other)
+        saved = (tab.tuning(), tab.kernel)
+        tab.autotune("vgh", P)
+        tab.set_point_order(ea.ORDER_LOCALITY)
+        lt = tab.tuning()
         w3, k3 = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 2, packed)
-        tab.set_point_order(tile[3])
-        res.setdefault("extra", {})["headline_" + ORDER_NAMES[other] + "_order"] = {
-            "evals_per_s": evals / (w3 * 1e-3), "kernel_ms": k3, "ms_per_step": w3,
-            "achieved_GBs": evals * bpe / (k3 * 1e-3) / 1e9}
-        log(f"[bench] FP32 vgh, {ORDER_NAMES[other]} order: {w3:.4f} ms/step wall")
+        loc = {"evals_per_s": evals / (w3 * 1e-3), "call_ms": k3, "ms_per_step": w3,
+               "tile": list(lt), "kernel": KERNELS[tab.kernel],
+               "algorithmic_GBs": evals * bpe / (k3 * 1e-3) / 1e9,
+               "achieved_TFLOPs": flops / (k3 * 1e-3) / 1e12,
+               "frac_valu": flops / (k3 * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS,
+               "speedup_vs_given_order": kern / k3}
+        lp = None
+        for name in ("eval3d_f32_vgh_48c_n256_p32768.locality.traffic.json",
+                     "eval3d_f32_vgh_48c_n256_p32768.locality.nt.traffic.json"):
+            cand = load_traffic(name)
+            tl = (cand or {}).get("tile", {})
+            if cand and tab.kernel == ea.KERNEL_VECTOR and \
+                    (tl.get("lane_bytes"), tl.get("block"), tl.get("store_policy")) == tuple(lt[:3]):
+                lp = cand
+                lp_name = name
+        if lp:
+            loc["counter_traffic_bytes"] = lp["bytes_per_launch"]
+            loc["counter_traffic_GBs"] = lp["bytes_per_launch"] / (k3 * 1e-3) / 1e9
+            loc["frac_hbm_counters"] = loc["counter_traffic_GBs"] / HBM_PEAK_GBS
+            loc["counter_traffic_source"] = f"profiles/{lp_name} (tile {lp.get('tile')})"
+        loc["note"] = ("call_ms includes the two sort kernels; algorithmic_GBs exceeds the HBM peak because "
+                       "most rows are L2 hits: counter traffic and VALU fraction describe this launch")
+        res.setdefault("extra", {})["headline_locality_order"] = loc
+        log(f"[bench] FP32 vgh, locality order: {w3:.4f} ms/step wall, {k3:.4f} ms/call")
+        tab.set_kernel(saved[1])
+        tab.set_tile(saved[0][0], saved[0][1])
+        tab.set_store_policy(saved[0][2])
+        tab.set_point_order(saved[0][3])
     # layout variant for the record
     if not packed and not args.no_extra:
         outs_p = tab.alloc_outputs("vgh", P, True)
         w2, k2 = time_batches(tab, "vgh", pos, P, outs_p, max(5, args.steps // 2), 2, True)
-        res.setdefault("extra", {})["f32_vgh_packed6"] = {
+        res.setdefault("extra", {})["f32_vgh_packed6_given_order"] = {
             "evals_per_s": evals / (w2 * 1e-3), "kernel_ms": k2,
             "achieved_GBs": evals * bytes_per_eval(4, "vgh", True, 4) / (k2 * 1e-3) / 1e9}
     return res
@@ -324,6 +470,11 @@ def precision(args, res):
         tab.set_point_order(ea.ORDER_GIVEN)
         _, kern_given = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 3, args.packed)
         tab.set_point_order(order_best)
+        # the same tuned launch with packed 6-entry Hessians (40 B of results per eval)
+        outs_p = tab.alloc_outputs("vgh", P, True)
+        _, kern_packed = time_batches(tab, "vgh", pos, P, outs_p, max(5, args.steps // 2), 3, True)
+        for b in outs_p[:3]:
+            b.free()
         # errors on the sample
         sp = ea.DeviceBuffer.from_array(sample)
         so = tab.alloc_outputs("vgh", len(sample))
@@ -338,7 +489,8 @@ def precision(args, res):
                        "rms_rel": float(np.sqrt(np.mean(d**2)) / np.sqrt(np.mean(ref**2)))}
         bpe = bytes_per_eval(cb, "vgh", args.packed, 4)
         rows[name] = {"evals_per_s": P * N / (wall * 1e-3), "kernel_ms": kern,
-                      "point_order": ("given", "locality")[order_best], "kernel_ms_given_order": kern_given,
+                      "point_order": ORDER_LABELS[order_best], "kernel_ms_given_order": kern_given,
+                      "kernel_ms_packed6": kern_packed,
                       "achieved_GBs": P * N * bpe / (kern * 1e-3) / 1e9,
                       "frac_hbm": P * N * bpe / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
                       "bytes_per_eval": bpe, "table_bytes": int(tab.bytes), "tile": list(tile),
@@ -352,6 +504,7 @@ def precision(args, res):
     rows["f16_speedup_vs_f32"] = rows["f32"]["kernel_ms"] / rows["f16"]["kernel_ms"]
     rows["bf16_speedup_vs_f32_given_order"] = (rows["f32"]["kernel_ms_given_order"]
                                                / rows["bf16"]["kernel_ms_given_order"])
+    rows["bf16_speedup_vs_f32_packed6"] = rows["f32"]["kernel_ms_packed6"] / rows["bf16"]["kernel_ms_packed6"]
     rows["note"] = ("speedup: each storage at its autotuned kernel, tile and point order; "
                     "_given_order: the same kernels on the points as given, where every stencil "
                     "row comes from memory and the element size decides; in the locality order "
@@ -405,18 +558,35 @@ def serving(args, res):
         # overstates what open (Poisson) arrivals can be served at.  Capacity
         # here is the highest Poisson rate whose p99 meets the SLO (bisection,
         # 1 s per probe); the latency rows are at 50% and 90% of it.
-        lo, hi = 0.0, cap_rps
-        for it in range(6):
-            mid = 0.5 * (lo + hi)
-            lt, _ = srv.run_load(ids, sinks, rate_hz=mid, duration_s=1.0, max_points=maxp, seed=100 + it)
-            if len(lt) and float(np.percentile(lt, 99)) <= SERVE_SLO_P99_S:
-                lo = mid
-            else:
-                hi = mid
+        mean_points = (s_after["evals"] - s_before["evals"]) / max(1, len(lat))   # evals per request
+
+        def capacity(slo):
+            lo, hi = 0.0, cap_rps
+            for it in range(6):
+                mid = 0.5 * (lo + hi)
+                lt, _ = srv.run_load(ids, sinks, rate_hz=mid, duration_s=1.0, max_points=maxp, seed=100 + it)
+                if len(lt) and float(np.percentile(lt, 99)) <= slo:
+                    lo = mid
+                else:
+                    hi = mid
+            return lo
+
+        lo = capacity(SERVE_SLO_P99_S)
         row["slo_p99_ms"] = SERVE_SLO_P99_S * 1e3
         row["capacity_req_per_s"] = lo
-        mean_points = (s_after["evals"] - s_before["evals"]) / max(1, len(lat))   # evals per request
         row["capacity_evals_per_s"] = lo * mean_points
+        slo_used = SERVE_SLO_P99_S
+        if lo == 0.0 and budget:
+            # Under a budget below the three tables' total, a request for a
+            # table that is not resident waits for an upload of at least
+            # 136 MB (the smallest table): no rate meets a 1 ms p99.  Report
+            # the capacity at the looser SLO as well.
+            slo_used = SERVE_SLO_SWAP_P99_S
+            lo = capacity(slo_used)
+            row["slo_swap_p99_ms"] = slo_used * 1e3
+            row["capacity_at_swap_slo_req_per_s"] = lo
+            row["capacity_at_swap_slo_evals_per_s"] = lo * mean_points
+        row["loads_at_slo_ms"] = slo_used * 1e3
         for frac in ((0.5, 0.9) if lo > 0 else ()):
             before = srv.stats()
             lat, wall = srv.run_load(ids, sinks, rate_hz=frac * lo, duration_s=args.serve_seconds,
@@ -428,6 +598,7 @@ def serving(args, res):
                 "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3),
                 "evals_per_s": ev / wall,
+                "admissions": after["admissions"] - before["admissions"],
                 "evictions": after["evictions"] - before["evictions"],
                 "batches": after["batches"] - before["batches"]}
             log(f"[bench] serving {budget_name} {int(frac * 100)}%: p50 "
