@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kThreads) fill_ratio(const esk::Params P,
     }
 
     const int mrem = P.nreal - e0;
-    if (mrem >= V) esk::store_block<A, V>(row + e0, acc, vec, false);
+    if (mrem >= V) esk::store_block<A, V>(row, e0, acc, vec, false);
     else if (mrem > 0) esk::store_partial<A, V>(row + e0, acc, mrem);
 #pragma unroll
     for (int e = 0; e < V; ++e)

@@ -165,7 +165,7 @@ __device__ __forceinline__ void eval12_chunk(const Params &P, int p, int ch) {
     A r[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) r[e] = acc[0][e];
-    if (full) store_block<A, V>(ov + e0, r, vec, false); else store_partial<A, V>(ov + e0, r, m);
+    if (full) store_block<A, V>(ov, e0, r, vec, false); else store_partial<A, V>(ov + e0, r, m);
   }
   if constexpr (MODE >= MODE_VG) {
     A r[DIM * V];
@@ -176,7 +176,7 @@ __device__ __forceinline__ void eval12_chunk(const Params &P, int p, int ch) {
         const int o = CPLX ? (e >> 1) * 2 * DIM + d * 2 + (e & 1) : e * DIM + d;
         r[o] = acc[1 + d][e];
       }
-    if (full) store_block<A, DIM * V>(og + DIM * e0, r, vec, false);
+    if (full) store_block<A, DIM * V>(og, DIM * e0, r, vec, false);
     else store_partial<A, DIM * V>(og + DIM * e0, r, DIM * m);
   }
   if constexpr (MODE == MODE_VGL || (MODE == MODE_VGH && DIM == 1)) {
@@ -187,7 +187,7 @@ __device__ __forceinline__ void eval12_chunk(const Params &P, int p, int ch) {
     for (int e = 0; e < V; ++e) r[e] = acc[LI][e];
     A *ol = oh ? oh : og + P.row_stride;      // pointer form without a lapl array: next row
     const bool lvec = vec && (((uintptr_t)ol & 15) == 0);
-    if (full) store_block<A, V>(ol + e0, r, lvec, false); else store_partial<A, V>(ol + e0, r, m);
+    if (full) store_block<A, V>(ol, e0, r, lvec, false); else store_partial<A, V>(ol + e0, r, m);
   }
   if constexpr (MODE == MODE_VGH && DIM == 2) {
     constexpr int full4[4] = {3, 4, 4, 5};
@@ -199,7 +199,7 @@ __device__ __forceinline__ void eval12_chunk(const Params &P, int p, int ch) {
         const int o = CPLX ? (e >> 1) * 8 + h * 2 + (e & 1) : e * 4 + h;
         r[o] = acc[full4[h]][e];
       }
-    if (full) store_block<A, 4 * V>(oh + 4 * e0, r, vec, false);
+    if (full) store_block<A, 4 * V>(oh, 4 * e0, r, vec, false);
     else store_partial<A, 4 * V>(oh + 4 * e0, r, 4 * m);
   }
 }

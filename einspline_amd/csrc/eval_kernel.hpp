@@ -236,28 +236,50 @@ template <int LB> struct Loader<f16s, LB> {
 // Result stores.  `nt` selects non-temporal (streaming) stores, which keep
 // the results from evicting the coefficient table from L2 and the Infinity
 // Cache.  They are used only where every store instruction of a wavefront
-// writes whole contiguous cache lines: a streaming store that covers part of
-// a line is far slower than a cached one.
-template <class T>
-__device__ __forceinline__ void st_out(T v, T *p, bool nt) {
-  if (nt) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
+// writes whole contiguous cache lines (a streaming store that covers part of
+// a line is far slower than a cached one) and the destination is a
+// wave-uniform base plus a per-lane offset: they are buffer stores with the
+// non-temporal hint (st_stream).  A runtime choice between the compiler's
+// nontemporal store builtin and a plain store does not work: the two stores
+// are merged into one and the hint is dropped.
 template <class T>
 __device__ __forceinline__ void st1(T *p, T v) { *p = v; }
 
-// Store K reals held in r[] to dst (contiguous).  `vec` selects 16-byte
-// stores (dst must then be 16-byte aligned and K*sizeof(A) a multiple of 16).
+// Streaming store of 4, 8 or 16 bytes at byte offset `off` (< 2^31) from a
+// wave-uniform base.
+template <class T>
+__device__ __forceinline__ void st_stream(T v, void *base, uint32_t off) {
+  // raw buffer: stride 0, no range limit below 2^31 bytes, 32-bit data format
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, 0x7fffffff, 0x00020000);
+  constexpr int kNT = 2;                    // cache policy: nt
+  if constexpr (sizeof(T) == 16) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, off, 0, kNT);
+  } else if constexpr (sizeof(T) == 8) {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, kNT);
+  } else {
+    static_assert(sizeof(T) == 4, "4, 8 or 16 bytes");
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, kNT);
+  }
+}
+
+// Store K reals held in r[] to base + off (contiguous).  `vec` selects
+// 16-byte stores (base + off must then be 16-byte aligned and K*sizeof(A) a
+// multiple of 16); `nt` (base wave-uniform) streaming stores where a store
+// instruction of the wavefront writes whole lines.
 template <class A, int K>
-__device__ __forceinline__ void store_block(A *dst, const A *r, bool vec, bool nt) {
+__device__ __forceinline__ void store_block(A *base, int off, const A *r, bool vec, bool nt) {
   constexpr int PER16 = 16 / sizeof(A);
+  A *const dst = base + off;
   if constexpr (K * sizeof(A) == 8) {
     if (vec) {
       typedef A v8 __attribute__((ext_vector_type(8 / sizeof(A))));
       v8 q;
 #pragma unroll
       for (int e = 0; e < K; ++e) q[e] = r[e];
-      st_out(q, (v8 *)dst, nt);       // 64 lanes x 8 bytes: whole lines
+      if (nt) st_stream(q, base, (uint32_t)off * (uint32_t)sizeof(A));   // 64 lanes x 8 bytes: whole lines
+      else *(v8 *)dst = q;
       return;
     }
   }
@@ -271,7 +293,8 @@ __device__ __forceinline__ void store_block(A *dst, const A *r, bool vec, bool n
         v16 q;
 #pragma unroll
         for (int e = 0; e < PER16; ++e) q[e] = r[c * PER16 + e];
-        st_out(q, (v16 *)dst + c, full_lines);
+        if (full_lines) st_stream(q, base, (uint32_t)off * (uint32_t)sizeof(A));
+        else ((v16 *)dst)[c] = q;
       }
       return;
     }
@@ -359,7 +382,8 @@ __device__ __forceinline__ A *lds_stage() {
 // LDS slice so that every store instruction writes 64 consecutive 16-byte
 // pieces (1 KB) instead of 16 bytes at a 4*K-byte stride per lane.
 template <class A, int K>
-__device__ __forceinline__ void wave_store(A *dst, const A *r, A *lds, bool nt) {
+__device__ __forceinline__ void wave_store(A *base, int off, const A *r, A *lds, bool nt) {
+  A *const dst = base + off;
   // largest piece (16 or 8 bytes) that divides a lane's K reals
   constexpr int PB = (K * sizeof(A)) % 16 == 0 ? 16 : 8;
   constexpr int PER = PB / (int)sizeof(A);
@@ -378,7 +402,8 @@ __device__ __forceinline__ void wave_store(A *dst, const A *r, A *lds, bool nt) 
   for (int c = 0; c < K / PER; ++c) {
     const int piece = c * 64 + lane;
     const vp q = *(const vp *)(lds + piece * PER);
-    st_out(q, (vp *)dst + piece, nt);
+    if (nt) st_stream(q, base, (uint32_t)(off + piece * PER) * (uint32_t)sizeof(A));
+    else ((vp *)dst)[piece] = q;
   }
   wave_sync();
 }
@@ -784,7 +809,8 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   const int e0 = ch * V;                 // first real of this lane
   const int m = P.nreal - e0;            // valid reals in this lane (may exceed V)
   const bool full = m >= V;
-  const bool nt = P.store == STORE_STAGED_NT;
+  // streaming stores need the wave-uniform output base of one point per wave
+  const bool nt = UNIF && P.store == STORE_STAGED_NT;
 
   // One point per wavefront, 64 full chunks and aligned outputs: the
   // wavefront's gradient and Hessian blocks are each one contiguous region,
@@ -806,13 +832,13 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
     const A sg = (MODE == MODE_V && P.sign) ? ((const A *)P.sign)[p] : (A)1;
 #pragma unroll
     for (int e = 0; e < V; ++e) r[e] = sg * acc[0][e];
-    if (full) store_block<A, V>(ov + e0, r, vec, nt); else store_partial<A, V>(ov + e0, r, m);
+    if (full) store_block<A, V>(ov, e0, r, vec, nt); else store_partial<A, V>(ov + e0, r, m);
   }
   if constexpr (MODE == MODE_VGL) {
     A r[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) r[e] = acc[4][e];
-    if (full) store_block<A, V>(oh + e0, r, vec, nt); else store_partial<A, V>(oh + e0, r, m);
+    if (full) store_block<A, V>(oh, e0, r, vec, nt); else store_partial<A, V>(oh + e0, r, m);
   }
   // Complex orbitals: the CPU layout interleaves re/im per entry.  Through
   // pointer arrays the reference's CUDA kernels treat a complex table as 2N
@@ -831,10 +857,10 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         r[o] = acc[1 + d][e];
       }
     if constexpr (STAGE) {
-      if (staged) wave_store<A, 3 * V>(og + 3 * wave_e0, r, lds, nt);
+      if (staged) wave_store<A, 3 * V>(og, 3 * wave_e0, r, lds, nt);
     }
     if (!staged) {
-      if (full) store_block<A, 3 * V>(og + 3 * e0, r, vec, nt);
+      if (full) store_block<A, 3 * V>(og, 3 * e0, r, vec, nt);
       else store_partial<A, 3 * V>(og + 3 * e0, r, 3 * m);
     }
   }
@@ -852,10 +878,10 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           r[o] = acc[pack6[h]][e];
         }
       if constexpr (STAGE) {
-        if (staged) wave_store<A, 6 * V>(oh + 6 * wave_e0, r, lds, nt);
+        if (staged) wave_store<A, 6 * V>(oh, 6 * wave_e0, r, lds, nt);
       }
       if (!staged) {
-        if (full) store_block<A, 6 * V>(oh + 6 * e0, r, vec, nt);
+        if (full) store_block<A, 6 * V>(oh, 6 * e0, r, vec, nt);
         else store_partial<A, 6 * V>(oh + 6 * e0, r, 6 * m);
       }
     } else {
@@ -868,10 +894,10 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
           r[o] = acc[full9[h]][e];
         }
       if constexpr (STAGE) {
-        if (staged) wave_store<A, 9 * V>(oh + 9 * wave_e0, r, lds, nt);
+        if (staged) wave_store<A, 9 * V>(oh, 9 * wave_e0, r, lds, nt);
       }
       if (!staged) {
-        if (full) store_block<A, 9 * V>(oh + 9 * e0, r, vec, nt);
+        if (full) store_block<A, 9 * V>(oh, 9 * e0, r, vec, nt);
         else store_partial<A, 9 * V>(oh + 9 * e0, r, 9 * m);
       }
     }
@@ -892,7 +918,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         r[o] = acc[sym27[h]][e];
       }
     const bool gvec = vec && ((uintptr_t)ogh & 15) == 0;
-    if (full) store_block<A, 27 * V>(ogh + 27 * e0, r, gvec, false);
+    if (full) store_block<A, 27 * V>(ogh, 27 * e0, r, gvec, false);
     else store_partial<A, 27 * V>(ogh + 27 * e0, r, 27 * m);
   }
 }

@@ -92,6 +92,8 @@ __device__ __forceinline__ bool mfma_point(const Params &P, int &p, int &slab, i
   slab = -1;                          // all slabs of the point
   if (P.slabs > 1) {                  // orbital slabs outermost: one slab per wavefront
     if (!slab_wave(P, blockIdx.x, blockDim.x >> 6, threadIdx.x >> 6, p, slab)) return false;
+    p = uniform(p);                   // wave-uniform by construction; said so to the compiler
+    slab = uniform(slab);
   } else {
     const unsigned b = logical_block(blockIdx.x, gridDim.x, P.xcd_runs);
     p = uniform((int)(b * (blockDim.x >> 6) + (threadIdx.x >> 6)));
@@ -136,16 +138,56 @@ __device__ __forceinline__ float by_order(int o, float w0, float w1, float w2) {
 }
 
 // Copy n floats of a staged block to dst: 16-byte pieces when dst is
-// aligned, streaming stores if nt.
-__device__ __forceinline__ void mfma_copy_out(int lane, const float *src, float *dst, int n, bool nt) {
+// aligned.
+__device__ __forceinline__ void mfma_copy_out(int lane, const float *src, float *dst, int n) {
   typedef float v4 __attribute__((ext_vector_type(4)));
   int done = 0;
   if (((uintptr_t)dst & 15) == 0) {
     const int n4 = n >> 2;
-    for (int i = lane; i < n4; i += 64) st_out(*(const v4 *)(src + 4 * i), (v4 *)(dst + 4 * i), nt);
+    for (int i = lane; i < n4; i += 64) *(v4 *)(dst + 4 * i) = *(const v4 *)(src + 4 * i);
     done = 4 * n4;
   }
   for (int i = done + lane; i < n; i += 64) dst[i] = src[i];
+}
+
+// Copy a full 128-orbital slab's staged v, gradient and Hessian blocks
+// (128 (4 + HS) floats, contiguous in the staging area) out with a fixed
+// sequence of store instructions, every one executed by all 64 lanes and
+// none behind a branch: the compiler then knows how many stores follow the
+// next slab's loads and waits for those loads only, not for the stores to
+// complete.  16-byte (and 8-byte) stores if the destinations are 16-byte
+// aligned, else one float per lane per instruction.
+template <int HS>
+__device__ __forceinline__ void mfma_copy_full(int lane, const float *stage, float *ov, float *og, float *oh,
+                                               bool vec, bool nt) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const float *sh = stage + 512;
+  if (vec && nt) {
+    const uint32_t l8 = 8u * lane, l16 = 16u * lane;
+    st_stream(*(const v2 *)(stage + 2 * lane), ov, l8);
+    st_stream(*(const v4 *)(stage + 128 + 4 * lane), og, l16);
+    st_stream(*(const v2 *)(stage + 384 + 2 * lane), og, 1024u + l8);
+#pragma unroll
+    for (int k = 0; k < HS * 128 / 256; ++k) st_stream(*(const v4 *)(sh + 256 * k + 4 * lane), oh, 1024u * k + l16);
+    if constexpr (HS * 128 % 256 != 0) st_stream(*(const v2 *)(sh + HS * 128 - 128 + 2 * lane), oh, 4u * (HS * 128 - 128) + l8);
+  } else if (vec) {
+    *(v2 *)(ov + 2 * lane) = *(const v2 *)(stage + 2 * lane);                   // v: 128 floats
+    *(v4 *)(og + 4 * lane) = *(const v4 *)(stage + 128 + 4 * lane);             // gradient: 384
+    *(v2 *)(og + 256 + 2 * lane) = *(const v2 *)(stage + 384 + 2 * lane);
+#pragma unroll
+    for (int k = 0; k < HS * 128 / 256; ++k)                                     // Hessian: 128 HS
+      *(v4 *)(oh + 256 * k + 4 * lane) = *(const v4 *)(sh + 256 * k + 4 * lane);
+    if constexpr (HS * 128 % 256 != 0)
+      *(v2 *)(oh + HS * 128 - 128 + 2 * lane) = *(const v2 *)(sh + HS * 128 - 128 + 2 * lane);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) ov[64 * k + lane] = stage[64 * k + lane];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) og[64 * k + lane] = stage[128 + 64 * k + lane];
+#pragma unroll
+    for (int k = 0; k < 2 * HS; ++k) oh[64 * k + lane] = sh[64 * k + lane];
+  }
 }
 
 // Register budget: three wavefronts per SIMD (at most 168 VGPRs; the
@@ -245,23 +287,31 @@ eval3d_mfma(const Params P) {
   // Eight stencil rows (K = 32 c + 8 g + jj) of the lane's eight orbitals
   // from o0: one lane offset (x row g/2, y rows 2 (g & 1) + .., orbital o0)
   // plus a wave-uniform offset per row.  Rows are padded to whole 32-byte
-  // pieces, so a lane whose o0 is inside the row reads inside it; other
-  // lanes read zeros.  (Buffer loads with the same offsets ran 30% slower
-  // in the locality order on MI355X.)
+  // pieces, so a lane whose o0 is inside the row reads inside it; a lane
+  // past the row reads the row's first orbitals instead.  Column n of a
+  // product depends on column n of B only, so those lanes change nothing
+  // but outputs of orbitals that are never stored, and every lane issues
+  // every load: no branch around them, so the compiler's wait counts stay
+  // exact and a half-slab in flight is not waited for early.  (Buffer loads
+  // with the same offsets ran 30% slower in the locality order on MI355X.)
+  // Each address is a wave-uniform base (the cell and the row, in scalar
+  // registers) plus the lane's 32-bit byte offset, so the 16 loads of a slab
+  // share one offset register; the host keeps the matrix kernel to tables
+  // whose offsets fit (two x planes, two y rows and a row below 2^31 bytes).
   constexpr int EB = (int)sizeof(S);
-  const long lane_off = ((g >> 1) * P.xs + 2 * (g & 1) * P.ys) * EB;
-  auto load_half = [&](uint4 (&R)[8], int s0, int c) {
+  const char *const cellb = (const char *)cell;
+  const uint32_t lane_off = (uint32_t)(((g >> 1) * P.xs + 2 * (g & 1) * P.ys) * EB);
+  auto load_half = [&](uint4 (&R)[8], int s0, int c) __attribute__((always_inline)) {
     const int o0 = s0 + 8 * r;
-    const bool have = o0 < row_reals;
-    const char *base = (const char *)cell + lane_off + (long)o0 * EB;
+    const uint32_t loff = lane_off + (uint32_t)((o0 < row_reals ? o0 : 0) * EB);
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
       const long soff = (2 * c * P.xs + (jj >> 2) * P.ys + (jj & 3) * P.zs) * EB;
-      R[jj] = have ? *(const uint4 *)(base + soff) : uint4{0u, 0u, 0u, 0u};
+      R[jj] = *(const uint4 *)(cellb + soff + loff);
     }
   };
   // Tile t takes element t of each row (an in-lane 8 x 8 transpose).
-  auto mma_half = [&](const uint4 (&R)[8], f4 (&acc)[8], int c) {
+  auto mma_half = [&](const uint4 (&R)[8], f4 (&acc)[8], int c) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int d = t >> 1, hi = t & 1;
@@ -278,20 +328,27 @@ eval3d_mfma(const Params P) {
       acc[t] = M::mma(c == 0 ? alo[0] : alo[1], bf, acc[t]);
     }
   };
-  // Software pipeline over half-slabs: the rows of the next half are in
-  // flight while the current one is multiplied and the slab is stored.
+  // Software pipeline over slabs: the first half's registers are refilled
+  // with the next slab's rows once its products are issued, the second's
+  // once the slab is staged in LDS, so the whole next slab is in flight
+  // while this one is stored.  The last
+  // slab is peeled off (no refill), which keeps every load unconditional.
   uint4 R0[8], R1[8];
   // all slabs of the point, or only the wavefront's slab (orbital slabs outermost)
   const int s_first = slab < 0 ? 0 : slab * 128;
   const int s_last = slab < 0 ? end : min(end, s_first + 128);
+  if (s_first >= s_last) return;
   load_half(R0, s_first, 0);
-  for (int s0 = s_first; s0 < s_last; s0 += 128) {
+  load_half(R1, s_first, 1);
+  // Slabs before the last are whole (s0 + 128 < s_last <= nreal) and take
+  // the fixed store sequence.
+  const bool aligned = (((uintptr_t)ov | (uintptr_t)og | (uintptr_t)oh) & 15) == 0;
+  auto slab_step = [&](int s0, bool more, bool full) __attribute__((always_inline)) {
     f4 acc[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    load_half(R1, s0, 1);
     mma_half(R0, acc, 0);
-    if (s0 + 128 < s_last) load_half(R0, s0 + 128, 0);
+    if (more) load_half(R0, s0 + 128, 0);
     mma_half(R1, acc, 1);
     // stage the slab (orbitals past the valid ones are staged but not
     // copied out)
@@ -316,12 +373,27 @@ eval3d_mfma(const Params P) {
           if (4 * g + e < Rows::NQ) h_base[Rows::HS * t + e] = acc[t][e] * sc[e];
     }
     wave_sync();
-    const int m = min(128, P.nreal - s0);             // valid orbitals of the slab
-    mfma_copy_out(lane, stage, ov + s0, m, nt);
-    mfma_copy_out(lane, stage + 128, og + 3 * s0, 3 * m, nt);
-    mfma_copy_out(lane, stage + 512, oh + Rows::HS * s0, Rows::HS * m, nt);
+    // the accumulators are staged: their registers take the next rows
+    if (more) load_half(R1, s0 + 128, 1);
+    if (full) {
+      mfma_copy_full<Rows::HS>(lane, stage, ov + s0, og + 3 * s0, oh + Rows::HS * s0, aligned, nt);
+    } else {
+      const int m = min(128, P.nreal - s0);           // valid orbitals of the slab
+      mfma_copy_out(lane, stage, ov + s0, m);
+      mfma_copy_out(lane, stage + 128, og + 3 * s0, 3 * m);
+      mfma_copy_out(lane, stage + 512, oh + Rows::HS * s0, Rows::HS * m);
+    }
     wave_sync();                                      // reads done before the next slab's writes
+  };
+  // The first slab is peeled off the loop too, so that on entry to it, as
+  // on every later pass, the next slab's loads are followed by exactly the
+  // slab's stores.
+  int s0 = s_first;
+  if (s0 + 128 < s_last) {
+    slab_step(s0, true, true);
+    for (s0 += 128; s0 + 128 < s_last; s0 += 128) slab_step(s0, true, true);
   }
+  slab_step(s0, false, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,7 +450,6 @@ eval3d_mfma32(const Params P) {
   const float *cell = (const float *)P.coefs + (long)ix * P.xs + (long)iy * P.ys + (long)iz * P.zs;
   const int row_reals = (int)P.zs;
   const int end = min(row_reals, P.nreal);
-  const bool nt = P.store == STORE_STAGED_NT;
   float *ov = (float *)P.out_v + (long)p * P.ld_v;
   float *og = (float *)P.out_g + (long)p * P.ld_g;
   float *oh = (float *)P.out_h + (long)p * P.ld_h;
@@ -432,9 +503,9 @@ eval3d_mfma32(const Params P) {
     }
     wave_sync();
     const int m = min(64, P.nreal - s0);              // valid orbitals of the slab
-    mfma_copy_out(lane, stage, ov + s0, m, nt);
-    mfma_copy_out(lane, stage + 64, og + 3 * s0, 3 * m, nt);
-    mfma_copy_out(lane, stage + 256, oh + Rows::HS * s0, Rows::HS * m, nt);
+    mfma_copy_out(lane, stage, ov + s0, m);
+    mfma_copy_out(lane, stage + 64, og + 3 * s0, 3 * m);
+    mfma_copy_out(lane, stage + 256, oh + Rows::HS * s0, Rows::HS * m);
     wave_sync();                                      // reads done before the next slab's writes
   }
 }

@@ -694,6 +694,11 @@ int esk_prepare(const multi_UBspline_3d_hip *t, int num, Launch &L, Params &P) {
   L.lane_bytes = lb;
   L.block = x->block ? x->block : 256;
   L.kernel = x->kernel;
+  // the 16-bit matrix kernel addresses a stencil row as the cell plus a
+  // 32-bit lane offset (one x plane, two y rows and an orbital offset)
+  if (L.kernel == EINSPLINE_KERNEL_MATRIX && eb == 2 &&
+      ((long long)t->x_stride + 2ll * t->y_stride + t->z_stride) * 2 >= (1ll << 31))
+    L.kernel = EINSPLINE_KERNEL_VECTOR;
   L.cplx = t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX;
   memset(&P, 0, sizeof P);
   P.coefs = t->coefs;
@@ -1279,12 +1284,14 @@ extern "C" int einspline_hip_stream_copy(void *dst, const void *src, size_t byte
   return e == hipSuccess ? 0 : hip_fail(e, "stream copy launch");
 }
 
-__global__ void __launch_bounds__(256) stream_fill_kernel(u32x4 *__restrict__ dst, unsigned v, size_t n,
-                                                          int nt) {
+// (two instantiations: a runtime choice between the nontemporal and the
+// plain store is merged by the compiler into one plain store)
+template <bool NT>
+__global__ void __launch_bounds__(256) stream_fill_kernel(u32x4 *__restrict__ dst, unsigned v, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const u32x4 q = {v, v, v, v};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    if (nt) __builtin_nontemporal_store(q, dst + i);
+    if constexpr (NT) __builtin_nontemporal_store(q, dst + i);
     else dst[i] = q;
   }
 }
@@ -1294,8 +1301,12 @@ extern "C" int einspline_hip_stream_fill(void *dst, unsigned value, size_t bytes
   if (!dst || bytes % 16) return fail(EINSPLINE_EINVAL, "fill needs 16-byte multiples");
   const size_t n = bytes / 16;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
-  hipLaunchKernelGGL(stream_fill_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
-                     (u32x4 *)dst, value, n, nontemporal ? 1 : 0);
+  if (nontemporal)
+    hipLaunchKernelGGL(stream_fill_kernel<true>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       (u32x4 *)dst, value, n);
+  else
+    hipLaunchKernelGGL(stream_fill_kernel<false>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       (u32x4 *)dst, value, n);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "stream fill launch");
 }
