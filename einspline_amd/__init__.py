@@ -219,6 +219,8 @@ _proto("einspline_autotune_points", _i, C.POINTER(multi_UBspline_3d_hip), _i, C.
 _proto("einspline_set_store_policy", _i, C.POINTER(multi_UBspline_3d_hip), _i)
 _proto("einspline_set_point_order", _i, C.POINTER(multi_UBspline_3d_hip), _i)
 _proto("einspline_set_kernel", _i, C.POINTER(multi_UBspline_3d_hip), _i)
+_proto("einspline_set_point_service", _i, _i, _i)
+_proto("einspline_point_service_resident", _i)
 _proto("einspline_get_kernel", _i, C.POINTER(multi_UBspline_3d_hip))
 _proto("einspline_get_tuning", _i, C.POINTER(multi_UBspline_3d_hip), C.POINTER(_i), C.POINTER(_i),
        C.POINTER(_i), C.POINTER(_i))
@@ -426,6 +428,18 @@ class Event:
 
 def synchronize():
     check(lib.einspline_hip_synchronize(), "device synchronize")
+
+
+def set_point_service(enable=True, idle_us=0):
+    """Serve single-point calls on 3-D tables with the resident polling
+    kernel (enable) or with one launch per call; idle_us > 0 sets how long
+    the kernel stays without calls.  Resident kernels leave first."""
+    check(lib.einspline_set_point_service(1 if enable else 0, int(idle_us)), "set_point_service")
+
+
+def point_service_resident() -> int:
+    """Devices on which the single-point kernel is resident now."""
+    return int(lib.einspline_point_service_resident())
 
 
 # ---------------------------------------------------------------------------

@@ -1152,18 +1152,27 @@ void mirror_release(void *spline, esk::MirrorRef *r) {
   esk::mirror_release(&einspline_host_of(spline)->lock, r, destroy_mirror_table);
 }
 
-// Evaluate one point into the thread's mapped staging memory and copy
-// `ncomp` result blocks of `elt`-byte result elements to the caller:
-// values (N), gradients (3N), Laplacian (N) or Hessian (9N).  One launch
-// and one stream synchronisation, no copies on the GPU side.  (Measured on
-// MI355X: an empty launch plus synchronisation takes 11 us, a launch that
-// writes 13 KB into mapped host memory 12.6 us, the same with a device
-// buffer and a copy back 14.4 us and with a position copy up as well
+// Evaluate one point and copy the result blocks to the caller: values (N),
+// gradients (3N), Laplacian (N) or Hessian (9N).  3-D tables go to the
+// resident polling kernel (point_service.hip) when it takes the call.
+// Otherwise the point is evaluated into the thread's mapped staging memory:
+// one launch and one stream synchronisation, no copies on the GPU side.
+// (Measured on MI355X: an empty launch plus synchronisation takes 11 us, a
+// launch that writes 13 KB into mapped host memory 12.6 us, the same with a
+// device buffer and a copy back 14.4 us and with a position copy up as well
 // 16.9 us; polling the stream with hipStreamQuery is slower, 19-23 us.)
 void eval_point(void *spline, const char *fn, int mode, double x, double y, double z,
                 void *vals, void *grads, void *lh, void *gh = nullptr) {
   esk::MirrorRef *mr = mirror_acquire(spline, fn);
   const multi_UBspline_3d_hip *t = (const multi_UBspline_3d_hip *)mr->table;
+  if (gh == nullptr) {
+    const int served = esk_point_service_eval(t, mode, x, y, z, vals, grads, lh);
+    if (served < 0) die(fn);
+    if (served > 0) {
+      mirror_release(spline, mr);
+      return;
+    }
+  }
   const size_t elt = (t->storage == EINSPLINE_STORE_F64 ? 8 : 4) *
                      ((t->tcode == SINGLE_COMPLEX || t->tcode == DOUBLE_COMPLEX) ? 2 : 1);
   const size_t N = (size_t)t->num_splines, dim = (size_t)t->dim;

@@ -25,3 +25,9 @@ void esk_order_end(EskOrderWs *w, hipStream_t s);
 // EINSPLINE_NOALLOC_MAX_POINTS points); shared by every such call on the GPU.
 int esk_order_begin_scratch(const esk::Params &P, const void *pos, bool pos_f32, int num, hipStream_t s,
                             const int **perm);
+
+// Single point of a 3-D table through the resident polling kernel
+// (point_service.hip): 1 when served (results copied out), 0 when the call
+// is not one it takes, negative on error.
+int esk_point_service_eval(const multi_UBspline_3d_hip *t, int mode, double x, double y, double z, void *vals,
+                           void *grads, void *lh);

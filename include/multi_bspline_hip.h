@@ -342,6 +342,21 @@ int einspline_autotune_points(multi_UBspline_3d_hip *spline, int mode, const dou
  * spline->coefs directly must call this before the next evaluation. */
 void einspline_hip_invalidate(void *spline);
 
+/* Calls on 3-D tables (v, vg, vgl, vgh; no reference counterpart: the
+ * reference evaluates on the CPU) are served by a resident one-wavefront
+ * kernel that polls a mailbox in pinned host memory, so a call costs no
+ * launch and no HIP call.  The kernel leaves after `idle_us` microseconds
+ * without a call (default 2000; 100 .. 1000000) and is started again by the
+ * next call.  While it is resident it holds one wavefront slot and a
+ * device-wide synchronisation waits for it to go idle.  enable = 0 makes
+ * every call one launch plus one stream synchronisation; idle_us <= 0
+ * keeps the current idle time.  Also set by the environment variables
+ * EINSPLINE_POINT_SERVICE=0 and EINSPLINE_POINT_SERVICE_IDLE_US at first
+ * use.  Resident kernels leave before the call returns. */
+int einspline_set_point_service(int enable, int idle_us);
+/* Number of devices on which the resident kernel is running now. */
+int einspline_point_service_resident(void);
+
 /* ---- minimal device runtime (so C and ctypes callers need no HIP) ------ */
 /* ---- tables solved on the GPU ------------------------------------------ */
 /* Build a 3-D device table from samples already in device memory: the solve
