@@ -457,8 +457,11 @@ struct StageLds {
 };
 
 // Evaluate chunk `ch` of point `p` (0 <= p < P.npoints, 0 <= ch < P.chunks).
-// With UNIF, p must be the same for the whole wavefront.
-template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS>
+// With UNIF, p must be the same for the whole wavefront.  PF: stencil steps
+// whose rows are requested ahead of the one being reduced (15 requests all
+// 64 rows at once: the shortest latency for one point, where throughput
+// does not matter); the results do not depend on it.
+template <class S, int LB, int MODE, bool CPLX, bool UNIF, class PosT, bool PTRS, int PF = ESK_PREFETCH>
 __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   using A = typename acc_of<S>::type;
   using L = Loader<S, LB>;
@@ -512,7 +515,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
   // rows of step st+PF are requested while step st is reduced, and a
   // scheduling barrier after each step stops the compiler from hoisting all
   // 64 row loads (which would need ~256 VGPRs and kill occupancy).
-  constexpr int PF = ESK_PREFETCH;
+  static_assert(PF >= 1 && PF <= 15, "prefetch depth");
   A rb[PF + 1][4][V];
 #pragma unroll
   for (int st = 0; st < PF; ++st) {

@@ -4,10 +4,11 @@
 // (eval_multi_UBspline_3d_{s,d,c,z}[_vg|_vgl|_vgh], src/multi_bspline_eval_*.h),
 // and callers invoke it in tight loops.  A kernel launch plus a stream
 // synchronisation alone costs about 11 us on MI355X, so calls on 3-D tables
-// are served by a resident kernel instead: one wavefront that polls a
-// mailbox in pinned host memory, evaluates the posted point with the batch
-// kernel's per-chunk code (eval_chunk, same arithmetic, so the same bits)
-// and writes the results straight into pinned host memory.  The host posts
+// are served by a resident kernel instead: one block whose first wavefront
+// polls a mailbox in pinned host memory; the block evaluates the posted
+// point with the batch kernel's per-chunk code (eval_chunk, same
+// arithmetic, so the same bits) and writes the results straight into
+// pinned host memory.  The host posts
 // a request by writing the mailbox and bumping its sequence number, then
 // spins on the "done" number; no HIP call is on that path.
 //
@@ -18,7 +19,7 @@
 // posted while the kernel was leaving is still served.  Every wait is
 // bounded on both sides.
 //
-// While the kernel is resident it holds one wavefront slot of one CU and a
+// While the kernel is resident it holds one block slot (four wavefronts) and a
 // device-wide synchronisation (hipDeviceSynchronize, and calls that imply
 // one) waits for it to go idle: EINSPLINE_POINT_SERVICE_IDLE_US sets the
 // idle time (default 2000 us) and EINSPLINE_POINT_SERVICE=0 turns the
@@ -51,59 +52,81 @@ struct alignas(64) Mailbox {
 constexpr int kParamWords = (int)(sizeof(esk::Params) / 4);
 static_assert(sizeof(esk::Params) % 4 == 0, "Params copied as 32-bit words");
 
+// The service evaluates with one real (one complex number) per lane and
+// all 64 stencil rows requested at once (half of them for complex FP64,
+// to stay within 128 row registers): the shortest latency for one point.
+template <class S, bool CPLX> struct ServeTile {
+  static constexpr int LB = (CPLX ? 2 : 1) * (int)sizeof(S);
+  static constexpr int PF = CPLX && sizeof(S) == 8 ? 7 : 15;
+};
+
 template <class S, int MODE, bool CPLX>
-__device__ __forceinline__ void serve_point(const esk::Params &P, int lane) {
-  for (int ch = lane; ch < P.chunks; ch += 64) esk::eval_chunk<S, 16, MODE, CPLX, false, double, false>(P, 0, ch);
+__device__ __forceinline__ void serve_point(const esk::Params &P, int tid) {
+  using T = ServeTile<S, CPLX>;
+  for (int ch = tid; ch < P.chunks; ch += 256)
+    esk::eval_chunk<S, T::LB, MODE, CPLX, false, double, false, T::PF>(P, 0, ch);
 }
 
 template <class S, bool CPLX>
-__device__ __forceinline__ void serve_mode(const esk::Params &P, int mode, int lane) {
+__device__ __forceinline__ void serve_mode(const esk::Params &P, int mode, int tid) {
   switch (mode) {
-    case esk::MODE_V: serve_point<S, esk::MODE_V, CPLX>(P, lane); break;
-    case esk::MODE_VG: serve_point<S, esk::MODE_VG, CPLX>(P, lane); break;
-    case esk::MODE_VGL: serve_point<S, esk::MODE_VGL, CPLX>(P, lane); break;
-    default: serve_point<S, esk::MODE_VGH, CPLX>(P, lane); break;
+    case esk::MODE_V: serve_point<S, esk::MODE_V, CPLX>(P, tid); break;
+    case esk::MODE_VG: serve_point<S, esk::MODE_VG, CPLX>(P, tid); break;
+    case esk::MODE_VGL: serve_point<S, esk::MODE_VGL, CPLX>(P, tid); break;
+    default: serve_point<S, esk::MODE_VGH, CPLX>(P, tid); break;
   }
 }
 
-// One wavefront.  `idle` is in ticks of the 100 MHz real-time counter.
-__global__ void __launch_bounds__(64) point_service_kernel(Mailbox *mb, unsigned long long idle) {
+// Four wavefronts: the first polls the mailbox and copies a request into
+// LDS, all four evaluate it.  `idle` is in ticks of the 100 MHz real-time
+// counter.
+__global__ void __launch_bounds__(256) point_service_kernel(Mailbox *mb, unsigned long long idle) {
   __shared__ unsigned words[kParamWords];
   __shared__ double spos[3];
-  const int lane = threadIdx.x;
+  __shared__ int cmd[2];                     // [0]: 1 serve, 0 leave; [1]: request kind
+  const int tid = threadIdx.x, lane = tid & 63;
   unsigned last = __hip_atomic_load(&mb->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   unsigned long long t_last = __builtin_amdgcn_s_memrealtime();
-  for (unsigned poll = 1;; ++poll) {
-    // one read of host memory per poll (each is a PCIe round trip)
-    const unsigned s = (unsigned)esk::uniform(
-        (int)__hip_atomic_load(&mb->req, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
-    if (s != last) {
-      const unsigned *src = (const unsigned *)&mb->P;
-      for (int i = lane; i < kParamWords; i += 64) words[i] = src[i];
-      if (lane < 3) spos[lane] = mb->pos[lane];
-      const int kind = esk::uniform(mb->kind);
-      esk::wave_sync();
-      esk::Params P;
-      __builtin_memcpy(&P, words, sizeof P);
-      P.pos = spos;
-      const int mode = kind & 3;
-      if (kind & 8) {
-        if (kind & 4) serve_mode<double, true>(P, mode, lane);
-        else serve_mode<double, false>(P, mode, lane);
-      } else {
-        if (kind & 4) serve_mode<float, true>(P, mode, lane);
-        else serve_mode<float, false>(P, mode, lane);
+  unsigned s = last;
+  for (;;) {
+    if (tid < 64) {
+      bool go = false;
+      for (unsigned poll = 1;; ++poll) {
+        // one read of host memory per poll (each is a PCIe round trip)
+        s = (unsigned)esk::uniform((int)__hip_atomic_load(&mb->req, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+        if (s != last) { go = true; break; }
+        if (__builtin_amdgcn_s_memrealtime() - t_last > idle) break;
+        if ((poll & 63) == 0 && __hip_atomic_load(&mb->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+        __builtin_amdgcn_s_sleep(2);
       }
-      // the results reach host memory before the sequence number does
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-      if (lane == 0) __hip_atomic_store(&mb->done, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      last = s;
-      t_last = __builtin_amdgcn_s_memrealtime();
-      continue;
+      if (go) {
+        const unsigned *src = (const unsigned *)&mb->P;
+        for (int i = lane; i < kParamWords; i += 64) words[i] = src[i];
+        if (lane < 3) spos[lane] = mb->pos[lane];
+        if (lane == 0) cmd[1] = mb->kind;
+      }
+      if (lane == 0) cmd[0] = go ? 1 : 0;
     }
-    if (__builtin_amdgcn_s_memrealtime() - t_last > idle) break;
-    if ((poll & 63) == 0 && __hip_atomic_load(&mb->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
-    __builtin_amdgcn_s_sleep(2);
+    __syncthreads();
+    if (!cmd[0]) break;                      // the same for the whole block
+    esk::Params P;
+    __builtin_memcpy(&P, words, sizeof P);
+    P.pos = spos;
+    const int kind = cmd[1], mode = kind & 3;
+    if (kind & 8) {
+      if (kind & 4) serve_mode<double, true>(P, mode, tid);
+      else serve_mode<double, false>(P, mode, tid);
+    } else {
+      if (kind & 4) serve_mode<float, true>(P, mode, tid);
+      else serve_mode<float, false>(P, mode, tid);
+    }
+    // every wavefront's results reach host memory before the sequence
+    // number does (and the request in LDS is no longer read)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&mb->done, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    last = s;
+    t_last = __builtin_amdgcn_s_memrealtime();
   }
 }
 
@@ -192,7 +215,7 @@ int setup(Service &v, size_t out_bytes) {
 
 int launch(Service &v, unsigned long long idle_ticks) {
   std::call_once(g_exit_once, [] { atexit(stop_all); });
-  hipLaunchKernelGGL(point_service_kernel, dim3(1), dim3(64), 0, v.stream, v.mb_dev, idle_ticks);
+  hipLaunchKernelGGL(point_service_kernel, dim3(1), dim3(256), 0, v.stream, v.mb_dev, idle_ticks);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return esk_hip_fail(e, "point-service launch");
   v.launched = true;
@@ -213,7 +236,6 @@ int esk_point_service_eval(const multi_UBspline_3d_hip *t, int mode, double x, d
   const bool f64 = t->storage == EINSPLINE_STORE_F64;
   if (!f64 && t->storage != EINSPLINE_STORE_F32) return 0;
   const size_t eb = f64 ? 8 : 4;
-  if (((size_t)t->row_reals * eb) % 16) return 0;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return esk_hip_fail(e, "hipGetDevice");
@@ -234,7 +256,7 @@ int esk_point_service_eval(const multi_UBspline_3d_hip *t, int mode, double x, d
   const size_t o_g = up(nv), o_h = o_g + up(ng), total = o_h + up(nh);
   if ((rc = setup(v, total))) return rc;
 
-  P.chunks = (int)(((size_t)t->row_reals * eb) / 16);
+  P.chunks = cplx ? t->row_reals / 2 : t->row_reals;          // one real (complex number) per lane
   P.npoints = 1;
   P.pos = nullptr;                                           // the kernel's copy of the position
   P.out_v = v.out_dev;
