@@ -352,23 +352,38 @@ def headline(args):
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, host cores)")
         res["cpu_baseline"] = cpu_baseline("s", host, pts, "vgh")
-    # the drop-in single-point API: one launch + result copy per call
+    # the drop-in single-point API: served by the resident polling kernel
+    # (no launch, no HIP call per call); also timed with one launch + stream
+    # synchronisation per call (the service off)
     if not args.no_extra:
         x = [tuple(float(c) for c in p) for p in pts[:400]]
         fn = ea.lib.eval_multi_UBspline_3d_s_vgh
         vb, gb, hb = np.zeros(N, np.float32), np.zeros(3 * N, np.float32), np.zeros(9 * N, np.float32)
         vp, gp, hp = vb.ctypes.data, gb.ctypes.data, hb.ctypes.data
         fn(host.ptr, *x[0], vp, gp, hp)               # builds the device mirror
-        t = []
-        for p in x:
-            t0 = time.perf_counter()
-            fn(host.ptr, p[0], p[1], p[2], vp, gp, hp)
-            t.append(time.perf_counter() - t0)
+
+        def point_calls():
+            fn(host.ptr, *x[0], vp, gp, hp)           # (re)starts the resident kernel
+            t = []
+            for p in x:
+                t0 = time.perf_counter()
+                fn(host.ptr, p[0], p[1], p[2], vp, gp, hp)
+                t.append(time.perf_counter() - t0)
+            return np.array(t) * 1e6
+
+        t_svc = point_calls()
+        ea.set_point_service(False)
+        t_launch = point_calls()
+        ea.set_point_service(True)
         res.setdefault("extra", {})["single_point_vgh_call_us"] = {
-            "median": float(np.median(t) * 1e6), "p99": float(np.percentile(t, 99) * 1e6),
+            "median": float(np.median(t_svc)), "p99": float(np.percentile(t_svc, 99)),
+            "launch_per_call_median": float(np.median(t_launch)),
+            "launch_per_call_p99": float(np.percentile(t_launch, 99)),
             "note": "eval_multi_UBspline_3d_s_vgh (256 orbitals, 13 results each) called through ctypes "
-                    "into preallocated host arrays; one launch + stream synchronisation per call (an empty "
-                    "launch + synchronisation alone takes about 11 us on this GPU)"}
+                    "into preallocated host arrays, 400 random points; median/p99: the resident polling "
+                    "kernel serves the call (no launch, no HIP call); launch_per_call: the service off, one "
+                    "launch + stream synchronisation per call (an empty launch + synchronisation alone "
+                    "takes about 11 us on this GPU)"}
     # the locality order: the library sorts the batch by cell first (two small
     # kernels inside the timed call); most stencil rows then come from L2
     if not args.no_extra:
@@ -557,7 +572,8 @@ def serving(args, res):
         # Closed loop keeps the queue full and batches maximally, so its rate
         # overstates what open (Poisson) arrivals can be served at.  Capacity
         # here is the highest Poisson rate whose p99 meets the SLO (bisection,
-        # 1 s per probe); the latency rows are at 50% and 90% of it.
+        # 1 s per probe, then confirmed on a longer run); the latency rows are
+        # at 50% and 90% of it.
         mean_points = (s_after["evals"] - s_before["evals"]) / max(1, len(lat))   # evals per request
 
         def capacity(slo):
@@ -569,6 +585,16 @@ def serving(args, res):
                     lo = mid
                 else:
                     hi = mid
+            # 1 s probes are short next to a table swap: confirm the rate on
+            # a run as long as the latency rows', backing off 10% at a time
+            for it in range(4):
+                if lo <= 0.0:
+                    break
+                lt, _ = srv.run_load(ids, sinks, rate_hz=lo, duration_s=args.serve_seconds, max_points=maxp,
+                                     seed=200 + it)
+                if len(lt) and float(np.percentile(lt, 99)) <= slo:
+                    break
+                lo *= 0.9
             return lo
 
         lo = capacity(SERVE_SLO_P99_S)
@@ -861,6 +887,8 @@ def main():
         summ["serving_p99_ms_at_50pct"] = r["load_50pct"]["p99_ms"]
         summ["serving_p99_ms_at_90pct"] = r["load_90pct"]["p99_ms"]
         summ["serving_capacity_evals_per_s_at_p99_1ms"] = r["capacity_evals_per_s"]
+    if "single_point_vgh_call_us" in ex:
+        summ["single_point_vgh_call_median_us"] = ex["single_point_vgh_call_us"]["median"]
     if "table_build" in ex:
         summ["gpu_table_build_speedup_64c_n256"] = ex["table_build"]["speedup_vs_host_solve_plus_upload"]
     if "determinant" in ex:

@@ -239,7 +239,7 @@ int einspline_server::init(const einspline_server_config *c) {
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     blocks[g] = prop.multiProcessorCount * per_cu;
   }
-  const size_t bytes = sizeof(BatchDesc) + (size_t)cfg.max_batch_points * 3 * sizeof(double);
+  const size_t bytes = sizeof(BatchDesc) + 16 + (size_t)cfg.max_batch_points * 3 * sizeof(double);
   slots.resize(cfg.slots);
   for (Slot &s : slots) {
     s.bytes = bytes;
@@ -431,7 +431,6 @@ void einspline_server::poll_completions() {
 
 // Form and launch one batch on slot s from the pending queue.
 bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
-  (void)lk;
   std::vector<char> held(tables.size(), 0), drain_now(tables.size(), 0);
   std::vector<Req *> take;
   int points = 0;
@@ -473,10 +472,8 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     t.draining = drain_now[i] != 0;
   }
   if (take.empty()) return false;
-  // group segments by kernel variant
-  BatchDesc *B = (BatchDesc *)s.hbuf;
-  double *hpos = (double *)(s.hbuf + sizeof(BatchDesc));
-  const double *dpos = (const double *)(s.dbuf + sizeof(BatchDesc));
+  // Kernel arguments of each request (its table is resident and, with the
+  // request counted in flight below, stays resident until the batch ends).
   std::vector<std::pair<int, Req *>> order;
   std::vector<esk::Params> params(take.size());
   for (size_t i = 0; i < take.size(); ++i) {
@@ -489,10 +486,27 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     const int g = storage_group(t.storage) + ((params[i].chunks % 64) ? 1 : 0);
     order.push_back({g, r});
     params[i].npoints = r->npts;
+    t.queued--;
+    t.inflight++;
+    t.last_use = ++use_clock;
   }
+  s.reqs = take;
+  s.busy = true;
+  st.batches++;
+  // The descriptor, the position copy and the launches run without the
+  // lock, so submitters are not held up by them.  Only this thread forms
+  // batches and touches a slot, and a busy slot's requests are not read by
+  // others until poll_completions (this thread) marks them done.
+  lk.unlock();
   std::vector<size_t> idx(take.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return order[a].first < order[b].first; });
+  // Layout of the copy: header, the batch's segments, then its positions
+  // (16-byte aligned); only these bytes go to the device.
+  BatchDesc *B = (BatchDesc *)s.hbuf;
+  const size_t pos_off = (offsetof(BatchDesc, seg) + take.size() * sizeof(Seg) + 15) & ~(size_t)15;
+  double *hpos = (double *)(s.hbuf + pos_off);
+  const double *dpos = (const double *)(s.dbuf + pos_off);
   memset(B, 0, offsetof(BatchDesc, seg));
   int off = 0;
   for (size_t k = 0; k < idx.size(); ++k) {
@@ -509,28 +523,23 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     sg.P.ld_g = 3l * sg.P.nreal;
     sg.P.ld_h = (cfg.hess_packed ? 6l : 9l) * sg.P.nreal;
     sg.P.hpacked = cfg.hess_packed ? 1 : 0;
-    tables[r->table].queued--;
     sg.wave0 = B->total_waves[g];
     sg.waves = (int)(((long)r->npts * sg.P.chunks + 63) / 64);
     B->total_waves[g] += sg.waves;
     B->nseg[g]++;
     off += r->npts;
-    tables[r->table].inflight++;
-    tables[r->table].last_use = ++use_clock;
   }
   unsigned mask = 0;
   for (int g = 0; g < kGroups; ++g)
     if (B->nseg[g]) mask |= 1u << g;
   hipGraphExec_t exec = graph_for(s, mask);
-  const size_t bytes = sizeof(BatchDesc) + sizeof(double) * 3 * (size_t)off;
+  const size_t bytes = pos_off + sizeof(double) * 3 * (size_t)off;
   hipError_t e = exec ? hipMemcpyAsync(s.dbuf, s.hbuf, bytes, hipMemcpyHostToDevice, s.stream)
                       : hipErrorInvalidValue;
   if (e == hipSuccess) e = hipGraphLaunch(exec, s.stream);
   if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+  lk.lock();
   if (e != hipSuccess) worker_error = esk_hip_fail(e, "server launch");
-  s.reqs = std::move(take);
-  s.busy = true;
-  st.batches++;
   return true;
 }
 
@@ -759,7 +768,12 @@ extern "C" int einspline_server_run_load(einspline_server *s, const einspline_lo
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::exponential_distribution<double> gap(ld->rate_hz > 0 ? ld->rate_hz : 1.0);
   const int maxp = ld->max_points > 0 ? ld->max_points : 2048;
-  std::vector<double> pos(3 * (size_t)maxp);
+  // Positions come from a pool of uniform random points drawn up front (a
+  // request takes a run of them at a random offset), so that generating the
+  // load costs little next to serving it.
+  const size_t pool_pts = std::max<size_t>(1 << 16, 8 * (size_t)maxp);
+  std::vector<double> pool(3 * pool_pts);
+  for (size_t i = 0; i < pool.size(); ++i) pool[i] = U(rng) * ld->box[i % 3];
   {
     std::lock_guard<std::mutex> g(s->mu);
     s->lat_log.clear();
@@ -790,8 +804,8 @@ extern "C" int einspline_server_run_load(einspline_server *s, const einspline_lo
     const int k = (int)(U(rng) * ld->ntables) % ld->ntables;
     // log-uniform point count on [1, maxp]
     const int np = std::min(maxp, std::max(1, (int)std::exp(U(rng) * std::log((double)maxp + 1.0))));
-    for (int i = 0; i < 3 * np; ++i) pos[i] = U(rng) * ld->box[i % 3];
-    int rc = submit_impl(s, ld->table_ids[k], pos.data(), np, ld->vals_d[k], ld->grads_d[k],
+    const size_t at = (size_t)(U(rng) * (double)(pool_pts - (size_t)np + 1)) % (pool_pts - (size_t)np + 1);
+    int rc = submit_impl(s, ld->table_ids[k], pool.data() + 3 * at, np, ld->vals_d[k], ld->grads_d[k],
                          ld->hess_d[k], nullptr);
     if (rc) return rc;
   }
