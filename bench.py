@@ -532,8 +532,9 @@ def precision(args, res):
 def serving(args, res):
     """Three resident tables (48^3 x 256 FP32, 64^3 x 512 BF16, 96^3 x 1024
     FP32), Poisson arrivals, log-uniform 1..2048 points per request, vgh.
-    Capacity is the highest Poisson rate with p99 <= 1 ms (bisection from the
-    closed-loop rate); latency at 50% and 90% of it."""
+    Capacity is the highest Poisson rate with p99 <= 1 ms (bisection below
+    the closed-loop rate, or twice it under a budget, then confirmed on two
+    longer runs); latency at 50% and 90% of it."""
     from einspline_amd.serve import Server
 
     t0 = time.perf_counter()
@@ -577,7 +578,11 @@ def serving(args, res):
         mean_points = (s_after["evals"] - s_before["evals"]) / max(1, len(lat))   # evals per request
 
         def capacity(slo):
-            lo, hi = 0.0, cap_rps
+            # The closed loop keeps at most slots x max_batch_requests requests
+            # queued, so under a budget, where latency is a swap cycle, its rate
+            # is that queue over the cycle and not the server's limit: search
+            # up to twice it there.
+            lo, hi = 0.0, cap_rps * (2.0 if budget else 1.0)
             for it in range(6):
                 mid = 0.5 * (lo + hi)
                 lt, _ = srv.run_load(ids, sinks, rate_hz=mid, duration_s=1.0, max_points=maxp, seed=100 + it)
@@ -586,18 +591,21 @@ def serving(args, res):
                 else:
                     hi = mid
             # 1 s probes are short next to a table swap: confirm the rate on
-            # a run as long as the latency rows', backing off 10% at a time
-            for it in range(4):
+            # two runs as long as the latency rows', backing off 10% at a time
+            for it in range(5):
                 if lo <= 0.0:
                     break
-                lt, _ = srv.run_load(ids, sinks, rate_hz=lo, duration_s=args.serve_seconds, max_points=maxp,
-                                     seed=200 + it)
-                if len(lt) and float(np.percentile(lt, 99)) <= slo:
-                    break
+                ok = True
+                for rep in range(2):
+                    lt, _ = srv.run_load(ids, sinks, rate_hz=lo, duration_s=args.serve_seconds,
+                                         max_points=maxp, seed=200 + 2 * it + rep)
+                    ok = ok and len(lt) > 0 and float(np.percentile(lt, 99)) <= slo
+                if ok:
+                    return lo, True
                 lo *= 0.9
-            return lo
+            return lo, False
 
-        lo = capacity(SERVE_SLO_P99_S)
+        lo, row["capacity_confirmed"] = capacity(SERVE_SLO_P99_S)
         row["slo_p99_ms"] = SERVE_SLO_P99_S * 1e3
         row["capacity_req_per_s"] = lo
         row["capacity_evals_per_s"] = lo * mean_points
@@ -608,7 +616,7 @@ def serving(args, res):
             # 136 MB (the smallest table): no rate meets a 1 ms p99.  Report
             # the capacity at the looser SLO as well.
             slo_used = SERVE_SLO_SWAP_P99_S
-            lo = capacity(slo_used)
+            lo, row["capacity_at_swap_slo_confirmed"] = capacity(slo_used)
             row["slo_swap_p99_ms"] = slo_used * 1e3
             row["capacity_at_swap_slo_req_per_s"] = lo
             row["capacity_at_swap_slo_evals_per_s"] = lo * mean_points

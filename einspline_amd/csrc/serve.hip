@@ -119,16 +119,18 @@ struct Table {
   int inflight = 0;                 // requests of this table on the GPU
   int queued = 0;                   // requests of this table in the pending queue
   bool draining = false;            // an absent table waits for this one to go idle
-  clk::time_point drain_since{};    // when it was first asked to drain
+  clk::time_point drain_since{};    // when it was first asked to drain: later requests wait
   int nsplines = 0;
 };
 
-// A resident table asked to make room for an absent one keeps serving the
-// requests queued for it for at most this long; then it gets no new work,
-// goes idle and is evicted.  Serving each table's queue in slices instead
-// of request by request keeps the number of uploads down when two tables
-// that do not fit together are both in demand.
-constexpr double kDrainSliceS = 2e-3;
+// A resident table asked to make room for an absent one ("draining") serves
+// the requests that were queued for it when the drain began, all of them and
+// no later ones; then it gets no new work, goes idle and is evicted.  Every
+// request is thus served in the first residency of its table that begins
+// after it arrives, however long that residency's queue is, and a residency
+// ends as soon as its queue at that moment is served.  Serving each table's
+// queue whole instead of request by request keeps the number of uploads down
+// when two tables that do not fit together are both in demand.
 
 struct Req {
   uint64_t id = 0;
@@ -294,7 +296,8 @@ hipGraphExec_t einspline_server::graph_for(Slot &s, unsigned mask) {
 // esk::plan_admission's: evict the cheapest set of idle tables that makes
 // room, or, when that set holds a table with work in flight (or in the
 // batch being formed: `held`), mark it in `drain_now` so that it is drained
-// (kDrainSliceS) and evicted once idle.  Only the oldest waiting request of
+// (it serves what was queued for it before the drain began) and evicted once
+// idle.  Only the oldest waiting request of
 // a pass may evict or drain (`may_evict`); later ones upload only into free
 // room.  Evicted tables are handed to the uploader thread, which frees them
 // outside the lock; uploads run there too, so the worker keeps launching
@@ -437,13 +440,13 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
   const auto now = clk::now();
   // Pass 1: requests of resident tables in arrival order, over the whole
   // queue, so a table serves everything queued for it; a table being
-  // drained only until its slice is over.  Only this thread removes from
-  // `pending`; submitters append.
+  // drained only what arrived before its drain began.  Only this thread
+  // removes from `pending`; submitters append.
   for (Req *r : pending) {
     if ((int)take.size() >= cfg.max_batch_requests || points >= cfg.max_batch_points) break;
     const Table &t = tables[r->table];
     if (!t.dev || points + r->npts > cfg.max_batch_points) continue;
-    if (t.draining && std::chrono::duration<double>(now - t.drain_since).count() > kDrainSliceS) {
+    if (t.draining && r->t0 > t.drain_since) {
       st.deferrals++;
       continue;
     }
