@@ -213,6 +213,8 @@ def load_traffic(name):
 # kind.  A candidate within 3% of the profiled tile keeps the profiled tile,
 # so that the committed counter summary describes the launch that is timed.
 HEADLINE_PROFILE = "eval3d_f32_vgh_48c_n256_p32768.given.traffic.json"
+# ... and of the same call in the locality order (extra.headline_locality_order)
+LOCALITY_PROFILE = "eval3d_f32_vgh_48c_n256_p32768.locality.traffic.json"
 
 
 def _time_config(tab, pos, P, outs, packed, reps=5, trials=3):
@@ -390,6 +392,22 @@ def headline(args):
         saved = (tab.tuning(), tab.kernel)
         tab.autotune("vgh", P)
         tab.set_point_order(ea.ORDER_LOCALITY)
+        # near-equal tiles swap places from box to box: keep the profiled
+        # vector tile when it is within 3% of the tuner's choice, so that the
+        # committed counter summary describes the launch that is timed
+        tuned = (tab.kernel, tab.tuning())
+        prof_loc = load_traffic(LOCALITY_PROFILE)
+        pt = (prof_loc or {}).get("tile", {})
+        ptile = (pt.get("lane_bytes"), pt.get("block"), pt.get("store_policy"))
+        if prof_loc and (tuned[0], tuned[1][:3]) != (ea.KERNEL_VECTOR, ptile):
+            t_tuned = _time_config(tab, pos, P, outs, packed)
+            tab.set_kernel(ea.KERNEL_VECTOR)
+            tab.set_tile(ptile[0], ptile[1])
+            tab.set_store_policy(ptile[2])
+            if _time_config(tab, pos, P, outs, packed) > 1.03 * t_tuned:
+                tab.set_kernel(tuned[0])
+                tab.set_tile(tuned[1][0], tuned[1][1])
+                tab.set_store_policy(tuned[1][2])
         lt = tab.tuning()
         w3, k3 = time_batches(tab, "vgh", pos, P, outs, max(5, args.steps // 2), 2, packed)
         loc = {"evals_per_s": evals / (w3 * 1e-3), "call_ms": k3, "ms_per_step": w3,
@@ -398,20 +416,11 @@ def headline(args):
                "achieved_TFLOPs": flops / (k3 * 1e-3) / 1e12,
                "frac_valu": flops / (k3 * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS,
                "speedup_vs_given_order": kern / k3}
-        lp = None
-        for name in ("eval3d_f32_vgh_48c_n256_p32768.locality.traffic.json",
-                     "eval3d_f32_vgh_48c_n256_p32768.locality.nt.traffic.json"):
-            cand = load_traffic(name)
-            tl = (cand or {}).get("tile", {})
-            if cand and tab.kernel == ea.KERNEL_VECTOR and \
-                    (tl.get("lane_bytes"), tl.get("block"), tl.get("store_policy")) == tuple(lt[:3]):
-                lp = cand
-                lp_name = name
-        if lp:
-            loc["counter_traffic_bytes"] = lp["bytes_per_launch"]
-            loc["counter_traffic_GBs"] = lp["bytes_per_launch"] / (k3 * 1e-3) / 1e9
+        if prof_loc and tab.kernel == ea.KERNEL_VECTOR and ptile == tuple(lt[:3]):
+            loc["counter_traffic_bytes"] = prof_loc["bytes_per_launch"]
+            loc["counter_traffic_GBs"] = prof_loc["bytes_per_launch"] / (k3 * 1e-3) / 1e9
             loc["frac_hbm_counters"] = loc["counter_traffic_GBs"] / HBM_PEAK_GBS
-            loc["counter_traffic_source"] = f"profiles/{lp_name} (tile {lp.get('tile')})"
+            loc["counter_traffic_source"] = f"profiles/{LOCALITY_PROFILE} (tile {prof_loc.get('tile')})"
         loc["note"] = ("call_ms includes the two sort kernels; algorithmic_GBs exceeds the HBM peak because "
                        "most rows are L2 hits: counter traffic and VALU fraction describe this launch")
         res.setdefault("extra", {})["headline_locality_order"] = loc

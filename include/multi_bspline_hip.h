@@ -343,12 +343,12 @@ int einspline_autotune_points(multi_UBspline_3d_hip *spline, int mode, const dou
 void einspline_hip_invalidate(void *spline);
 
 /* Calls on 3-D tables (v, vg, vgl, vgh; no reference counterpart: the
- * reference evaluates on the CPU) are served by a resident one-wavefront
- * kernel that polls a mailbox in pinned host memory, so a call costs no
- * launch and no HIP call.  The kernel leaves after `idle_us` microseconds
- * without a call (default 2000; 100 .. 1000000) and is started again by the
- * next call.  While it is resident it holds one wavefront slot and a
- * device-wide synchronisation waits for it to go idle.  enable = 0 makes
+ * reference evaluates on the CPU) are served by a resident kernel (one block
+ * of four wavefronts) that polls a mailbox in pinned host memory, so a call
+ * costs no launch and no HIP call.  The kernel leaves after `idle_us`
+ * microseconds without a call (default 2000; 100 .. 1000000) and is started
+ * again by the next call.  While it is resident it holds one block slot and
+ * a device-wide synchronisation waits for it to go idle.  enable = 0 makes
  * every call one launch plus one stream synchronisation; idle_us <= 0
  * keeps the current idle time.  Also set by the environment variables
  * EINSPLINE_POINT_SERVICE=0 and EINSPLINE_POINT_SERVICE_IDLE_US at first
