@@ -46,6 +46,23 @@ struct Seg {
   int waves;
 };
 
+// Re-admission copy: device memory from pinned host memory through its
+// device address, 16 bytes per lane and then any tail bytes.  A copy kernel
+// runs at the PCIe rate whatever DMA engine a stream would be given (on
+// MI355X, DMA copies of re-admitted tables ran at 57 or 30 GB/s from one
+// process to the next); it takes kPullBlocks workgroups while it runs.
+constexpr int kPullBlocks = 128;
+__global__ void __launch_bounds__(256) pull_table(const unsigned char *__restrict__ src,
+                                                  unsigned char *__restrict__ dst, size_t bytes) {
+  const size_t n16 = bytes / 16;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint4 *s16 = (const uint4 *)src;
+  uint4 *d16 = (uint4 *)dst;
+  for (size_t i = first; i < n16; i += stride) d16[i] = s16[i];
+  for (size_t i = 16 * n16 + first; i < bytes; i += stride) dst[i] = src[i];
+}
+
 struct BatchDesc {
   int nseg[kGroups];
   int seg0[kGroups];
@@ -169,6 +186,8 @@ struct einspline_server {
   std::vector<Table> tables;
   std::vector<Slot> slots;
   std::thread worker, uploader;
+  // Re-admissions run pull_table on a non-blocking stream of their own.
+  hipStream_t upload_stream = nullptr;
   std::deque<int> uploads;          // tables waiting for the uploader thread
   std::vector<multi_UBspline_3d_hip *> frees;   // evicted device tables, freed by the uploader
   std::condition_variable cv_upload;
@@ -251,6 +270,8 @@ int einspline_server::init(const einspline_server_config *c) {
         (e = hipMalloc((void **)&s.dbuf, bytes)) != hipSuccess)
       return esk_hip_fail(e, "server slot setup");
   }
+  if ((e = hipStreamCreateWithFlags(&upload_stream, hipStreamNonBlocking)) != hipSuccess)
+    return esk_hip_fail(e, "server upload stream");
   worker = std::thread([this] { run(); });
   uploader = std::thread([this] { upload_loop(); });
   return EINSPLINE_OK;
@@ -372,9 +393,11 @@ void einspline_server::upload_loop() {
     void *new_pinned = nullptr;
     if (pinned) {
       // re-admission: the converted table comes straight from pinned memory
-      void *dc = nullptr;
-      if (hipMalloc(&dc, meta.bytes) == hipSuccess) {
-        if (hipMemcpy(dc, pinned, meta.bytes, hipMemcpyHostToDevice) == hipSuccess)
+      void *dc = nullptr, *src = nullptr;
+      if (hipHostGetDevicePointer(&src, pinned, 0) == hipSuccess && hipMalloc(&dc, meta.bytes) == hipSuccess) {
+        hipLaunchKernelGGL(pull_table, dim3(kPullBlocks), dim3(256), 0, upload_stream,
+                           (const unsigned char *)src, (unsigned char *)dc, meta.bytes);
+        if (hipGetLastError() == hipSuccess && hipStreamSynchronize(upload_stream) == hipSuccess)
           d = esk_table_wrap(&meta, dc);
         if (!d) (void)hipFree(dc);
       }
@@ -581,7 +604,7 @@ extern "C" einspline_server *einspline_server_create(const einspline_server_conf
   einspline_server_config def{};
   einspline_server *s = new einspline_server();
   if (s->init(cfg ? cfg : &def) != EINSPLINE_OK) {
-    delete s;
+    einspline_server_destroy(s);    // releases whatever init set up (no thread runs yet)
     return nullptr;
   }
   return s;
@@ -608,6 +631,7 @@ extern "C" void einspline_server_destroy(einspline_server *s) {
     if (sl.hbuf) (void)hipHostFree(sl.hbuf);
     if (sl.dbuf) (void)hipFree(sl.dbuf);
   }
+  if (s->upload_stream) (void)hipStreamDestroy(s->upload_stream);
   for (multi_UBspline_3d_hip *d : s->frees) destroy_multi_UBspline_3d_hip(d);
   for (Table &t : s->tables) {
     if (t.dev) destroy_multi_UBspline_3d_hip(t.dev);

@@ -382,7 +382,11 @@ def test_headline_at_benchmarked_tile_and_locality_order():
     o.coefs[...] = h.coefs()
     idx = rng.choice(P, 512, replace=False)
     ref = o.eval_batch("vgh", pts[idx])
-    configs = [("autotuned", None), ("16 B x 256, staged stores, locality", (16, 256, 1, ea.ORDER_LOCALITY))]
+    configs = [("autotuned", None),
+               ("headline tile: 8 B x 256, direct stores, given order", (8, 256, 2, ea.ORDER_GIVEN)),
+               ("locality profile tile: 16 B x 64, staged streaming stores, locality",
+                (16, 64, 0, ea.ORDER_LOCALITY)),
+               ("16 B x 256, staged stores, locality", (16, 256, 1, ea.ORDER_LOCALITY))]
     for name, cfg in configs:
         if cfg is None:
             tab.autotune("vgh", P)
