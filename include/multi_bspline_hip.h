@@ -347,8 +347,10 @@ void einspline_hip_invalidate(void *spline);
  * of four wavefronts) that polls a mailbox in pinned host memory, so a call
  * costs no launch and no HIP call.  The kernel leaves after `idle_us`
  * microseconds without a call (default 2000; 100 .. 1000000) and is started
- * again by the next call.  While it is resident it holds one block slot and
- * a device-wide synchronisation waits for it to go idle.  enable = 0 makes
+ * again by the next call.  While it is resident it holds one block slot, a
+ * device-wide synchronisation waits for it to go idle, and so does work on
+ * any stream that the HIP runtime maps to the same hardware queue (streams
+ * share queues once GPU_MAX_HW_QUEUES are in use).  enable = 0 makes
  * every call one launch plus one stream synchronisation; idle_us <= 0
  * keeps the current idle time.  Also set by the environment variables
  * EINSPLINE_POINT_SERVICE=0 and EINSPLINE_POINT_SERVICE_IDLE_US at first
