@@ -33,10 +33,12 @@ def alone(host, storage, pts):
     return tab.download("vgh", P, outs)
 
 
-def run_mix(srv, ids, hosts, storages, rng, nreq=120):
+def run_mix(srv, ids, hosts, storages, rng, nreq=120, table=None):
+    """Submit nreq requests (for random tables, or all for `table`), wait for
+    all of them and check each against the same points evaluated alone."""
     reqs = []
     for r in range(nreq):
-        k = int(rng.integers(0, 3))
+        k = int(rng.integers(0, 3)) if table is None else table
         n = int(rng.integers(1, 300))
         pts = rng.random((n, 3))
         N = hosts[k].N
@@ -88,10 +90,19 @@ def test_eviction_under_budget(rng):
     budget = sorted(sizes)[-1] + sorted(sizes)[-2]     # room for two of the three
     srv = Server(hbm_budget=budget, slots=2, max_batch_points=1024, max_batch_requests=8)
     ids = [srv.register(h, s) for h, s in zip(hosts, storages)]
-    run_mix(srv, ids, hosts, storages, rng, nreq=60)
+    # Each table in turn, twice, every request waited for before the next
+    # table's: the third table evicts one of the first two, and the second
+    # pass finds at least one table absent again, whatever the victims.
+    for _ in range(2):
+        for k in range(3):
+            run_mix(srv, ids, hosts, storages, rng, nreq=10, table=k)
+            assert srv.stats()["resident_bytes"] <= budget
     st = srv.stats()
-    assert st["evictions"] > 0 and st["admissions"] > 3
-    assert st["resident_bytes"] <= budget
+    assert st["evictions"] >= 2 and st["admissions"] >= 4
+    assert st["completed"] == 60
+    # then a mix of all three, queued together
+    run_mix(srv, ids, hosts, storages, rng, nreq=60)
+    assert srv.stats()["resident_bytes"] <= budget
     srv.close()
 
 
