@@ -36,10 +36,13 @@ HBM_PEAK_GBS = 8000.0
 # FP32 vector FMA rate without the packed two-wide forms (which this build
 # keeps off): 256 CUs x 4 SIMDs x 16 lanes x 2 flops x 2.4 GHz
 VALU_FP32_PEAK_TFLOPS = 78.6
-# flops of one (point, orbital) evaluation (z reduction + sum-factorised x/y
-# accumulation; 352 FMAs for vgh)
-FLOPS_PER_EVAL = {"v": 2 * (4 * 16 + 16), "vg": 2 * (8 * 16 + 4 * 16), "vgl": 2 * (12 * 16 + 7 * 16),
-                  "vgh": 704}
+# flops of one (point, orbital) evaluation of an FP32 table, as the vector
+# kernel contracts it: z first (per (i, j): 1 multiply + 3 FMAs per z
+# partial), then y into per-x-row partials (per x row: 1 multiply + 3 FMAs
+# each), then x into the accumulators (1 FMA per accumulator and x row); an
+# FMA counts two flops.  vgh: 336 + 168 + 80 = 584 flops in 328 operations.
+FLOPS_PER_EVAL = {"v": 16 * 7 + 4 * 7 + 4 * 2, "vg": 16 * 14 + 4 * 21 + 4 * 8,
+                  "vgl": 16 * 21 + 4 * 35 + 4 * 14, "vgh": 16 * 21 + 4 * 42 + 4 * 20}
 SERVE_SLO_P99_S = 1e-3          # serving capacity is quoted at this p99 latency
 SERVE_SLO_SWAP_P99_S = 20e-3    # ... and, when tables must be swapped, also at this one
 METRIC = "orbital-vgh evals/sec + %HBM roofline, FP32 vs BF16 table; serving p99 latency"

@@ -293,9 +293,8 @@ __global__ void __launch_bounds__(kThreads) fill_ratio(const esk::Params P,
     A rb[2][4][V];
 #pragma unroll
     for (int k = 0; k < 4; ++k) L::load(base + k * zs, rb[0][k]);
-#if ESK_YX
+    constexpr bool YX = esk::contract_yx<S>::value;   // as the value entry point
     A tq[V];
-#endif
 #pragma unroll
     for (int st = 0; st < 16; ++st) {
       const int i = st >> 2, j = st & 3;
@@ -313,18 +312,18 @@ __global__ void __launch_bounds__(kThreads) fill_ratio(const esk::Params P,
         s0[e] = esk::fmaA(c[2], r[2][e], s0[e]);
         s0[e] = esk::fmaA(c[3], r[3][e], s0[e]);
       }
-#if ESK_YX
+      if constexpr (YX) {
 #pragma unroll
-      for (int e = 0; e < V; ++e) tq[e] = j == 0 ? b[j] * s0[e] : esk::fmaA(b[j], s0[e], tq[e]);
-      if (j == 3) {
+        for (int e = 0; e < V; ++e) tq[e] = j == 0 ? b[j] * s0[e] : esk::fmaA(b[j], s0[e], tq[e]);
+        if (j == 3) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) acc[e] = esk::fmaA(a[i], tq[e], acc[e]);
+          for (int e = 0; e < V; ++e) acc[e] = esk::fmaA(a[i], tq[e], acc[e]);
+        }
+      } else {
+        const A ab = a[i] * b[j];
+#pragma unroll
+        for (int e = 0; e < V; ++e) acc[e] = esk::fmaA(ab, s0[e], acc[e]);
       }
-#else
-      const A ab = a[i] * b[j];
-#pragma unroll
-      for (int e = 0; e < V; ++e) acc[e] = esk::fmaA(ab, s0[e], acc[e]);
-#endif
 #pragma unroll
       for (int e = 0; e < V; ++e) esk::pin(acc[e]);
       __builtin_amdgcn_sched_barrier(0);

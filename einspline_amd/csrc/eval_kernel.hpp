@@ -28,12 +28,17 @@ namespace esk {
 #ifndef ESK_WAVES_PER_EU
 #define ESK_WAVES_PER_EU 4
 #endif
-// ESK_YX: contract y before x.  For each x row i the z-reduced partials of
-// the four y rows are combined with the y basis into per-i partials, which
-// the x basis then folds into the accumulators (fewer FMAs, and no products
-// of two uniform basis values, at the cost of the per-i partials' registers).
+// Contracting y before x: for each x row i the z-reduced partials of the
+// four y rows are combined with the y basis into per-i partials, which the x
+// basis then folds into the accumulators.  That takes fewer FMAs and no
+// products of two uniform basis values, at the cost of the per-i partials'
+// registers.  On MI355X it is 2-4% faster for FP32 tables where the launch
+// is bound by FP32 issue (48^3 x 256, locality order) and no slower
+// elsewhere, but 2-6% slower for the 16-bit tables' vector kernel, so it is
+// used for 32- and 64-bit tables.  ESK_YX = 0 or 1 forces either
+// contraction.
 #ifndef ESK_YX
-#define ESK_YX 0
+#define ESK_YX -1
 #endif
 #ifndef ESK_PREFETCH
 #define ESK_PREFETCH 1
@@ -53,6 +58,11 @@ struct f16s  { uint16_t b; };
 template <class S> struct acc_of        { using type = S; };
 template <>        struct acc_of<bf16s> { using type = float; };
 template <>        struct acc_of<f16s>  { using type = float; };
+
+// Whether tables of storage type S contract y before x (see ESK_YX).
+template <class S> struct contract_yx {
+  static constexpr bool value = ESK_YX < 0 ? sizeof(S) >= 4 : ESK_YX != 0;
+};
 
 // Kernel arguments (passed by value; ~200 bytes).
 struct Params {
@@ -433,8 +443,8 @@ struct Occupancy {
   // double weights overflow the scalar register file, so half of them are
   // counted as vector registers even for one point per wavefront
   // per-x-row y partials of the y-then-x contraction
-  static constexpr int NT = !ESK_YX ? 0 : MODE == MODE_V ? 1 : MODE == MODE_VG ? 3 : MODE == MODE_VGL ? 5
-                          : MODE == MODE_VGHGH ? 10 : 6;
+  static constexpr int NT = !contract_yx<S>::value ? 0 : MODE == MODE_V ? 1 : MODE == MODE_VG ? 3
+                          : MODE == MODE_VGL ? 5 : MODE == MODE_VGHGH ? 10 : 6;
   static constexpr int regs = NACC * V * W + (ESK_PREFETCH + 1) * 4 * V * W + 3 * V * W + NT * V * W +
                               (UNIF ? (W == 2 ? NW : 0) : NW * W) + 24;
   static constexpr int waves = regs <= 116 ? ESK_WAVES_PER_EU : regs <= 160 ? 3 : regs <= 248 ? 2 : 1;
@@ -522,12 +532,11 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) load_row(st, k, rb[st][k]);
   }
-#if ESK_YX
+  constexpr bool YX = contract_yx<S>::value;
   // y partials per x row: b s0, db s0, d2b s0, b s1, db s1, b s2 (+ d3b s0,
-  // d2b s1, db s2, b s3 for third derivatives)
+  // d2b s1, db s2, b s3 for third derivatives); unused without YX
   constexpr int NT = MODE == MODE_V ? 1 : MODE == MODE_VG ? 3 : MODE == MODE_VGL ? 5 : THIRD ? 10 : 6;
   A tq[NT][V];
-#endif
 #pragma unroll
   for (int st = 0; st < 16; ++st) {
     const int i = st >> 2, j = st & 3;
@@ -562,8 +571,8 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         s2[e] = fmaA(d2c[3], r[3][e], s2[e]);
       }
     }
-#if ESK_YX
-    {
+    if constexpr (YX) {
+      {
       // y step: fold row j into the per-i partials (basis values straight from
       // scalar registers); the first row of each i initialises them
       constexpr A D3[4] = {(A)-1, (A)3, (A)-3, (A)1};
@@ -658,7 +667,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
     for (int q = 0; q < NT; ++q)
 #pragma unroll
       for (int e = 0; e < V; ++e) pin(tq[q][e]);
-#else
+    } else {
     const A ab = a[i] * b[j];
     if constexpr (MODE == MODE_V) {
 #pragma unroll
@@ -714,7 +723,7 @@ __device__ __forceinline__ void eval_chunk(const Params &P, int p, int ch) {
         }
       }
     }
-#endif
+    }
     // Materialise the accumulators here: without this the optimiser sinks
     // the reductions past later loads and keeps every loaded row alive.
 #pragma unroll
