@@ -192,7 +192,16 @@ __device__ __forceinline__ void mfma_copy_full(int lane, const float *stage, flo
 
 // Register budget: three wavefronts per SIMD (at most 168 VGPRs; the
 // kernel needs about 150 without spills, while four would spill).
-template <class S, bool PACKED>
+//
+// STREAM: the launch guarantees that every whole slab's destinations are
+// 16-byte aligned and asks for streaming stores, so whole slabs take the one
+// fixed store sequence.  With the store variants chosen at run time inside
+// the slab loop, the compiler's count of outstanding memory operations at
+// the loop head is the most conservative over the variants' paths, and it
+// then waits for the previous slab's stores (vector-memory counts include
+// stores) before the slab's last rows: a store round trip per slab.  With
+// one path the waits are exact and the stores stay in flight.
+template <class S, bool PACKED, bool STREAM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 eval3d_mfma(const Params P) {
   using M = Mfma16<S>;
@@ -376,7 +385,10 @@ eval3d_mfma(const Params P) {
     // the accumulators are staged: their registers take the next rows
     if (more) load_half(R1, s0 + 128, 1);
     if (full) {
-      mfma_copy_full<Rows::HS>(lane, stage, ov + s0, og + 3 * s0, oh + Rows::HS * s0, aligned, nt);
+      if constexpr (STREAM)
+        mfma_copy_full<Rows::HS>(lane, stage, ov + s0, og + 3 * s0, oh + Rows::HS * s0, true, true);
+      else
+        mfma_copy_full<Rows::HS>(lane, stage, ov + s0, og + 3 * s0, oh + Rows::HS * s0, aligned, nt);
     } else {
       const int m = min(128, P.nreal - s0);           // valid orbitals of the slab
       mfma_copy_out(lane, stage, ov + s0, m);
@@ -387,13 +399,15 @@ eval3d_mfma(const Params P) {
   };
   // The first slab is peeled off the loop too, so that on entry to it, as
   // on every later pass, the next slab's loads are followed by exactly the
-  // slab's stores.
+  // slab's stores.  The last slab takes the fixed (streaming) sequence too
+  // when it is whole: with cached stores there, a quarter of a 512-orbital
+  // point's results evicted table rows from L2.
   int s0 = s_first;
   if (s0 + 128 < s_last) {
     slab_step(s0, true, true);
     for (s0 += 128; s0 + 128 < s_last; s0 += 128) slab_step(s0, true, true);
   }
-  slab_step(s0, false, false);
+  slab_step(s0, false, s0 + 128 <= P.nreal);
 }
 
 // ---------------------------------------------------------------------------
