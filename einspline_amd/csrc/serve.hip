@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <random>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -160,6 +161,7 @@ struct Req {
   bool done = false;
   bool collect = true;              // false: nobody waits, drop at completion
   bool taken = false;               // in the batch being formed
+  bool failed = false;              // its batch failed: the outputs are not valid
 };
 
 struct Slot {
@@ -193,7 +195,8 @@ struct einspline_server {
   std::condition_variable cv_upload;
   bool stop = false;
   bool stop_uploader = false;       // set after the worker has drained
-  int worker_error = 0;
+  int worker_error = 0;             // first failure of a batch or an upload: sticky
+  std::string worker_msg;           // its message (errors are recorded per thread)
   uint64_t next_id = 1, use_clock = 0;
   size_t budget = 0, used = 0;
   einspline_server_stats st{};
@@ -206,6 +209,15 @@ struct einspline_server {
   bool launch_batch(Slot &s, std::unique_lock<std::mutex> &lk);
   int admit(int tid, bool may_evict, const std::vector<char> &held, std::vector<char> &drain_now);
   hipGraphExec_t graph_for(Slot &s, unsigned mask);
+  // Record the first failure (the calling thread's last error message goes
+  // with it), under the lock.
+  void set_error(int code) {
+    if (worker_error) return;
+    worker_error = code;
+    worker_msg = einspline_hip_last_error();
+  }
+  // Hand the sticky failure to a caller's thread: its code and message.
+  int error_for_caller() const { return esk_fail(worker_error, "%s", worker_msg.c_str()); }
 };
 
 // ---------------------------------------------------------------------------
@@ -391,19 +403,25 @@ void einspline_server::upload_loop() {
     lk.unlock();
     multi_UBspline_3d_hip *d = nullptr;
     void *new_pinned = nullptr;
+    int err = EINSPLINE_OK;
     if (pinned) {
       // re-admission: the converted table comes straight from pinned memory
       void *dc = nullptr, *src = nullptr;
-      if (hipHostGetDevicePointer(&src, pinned, 0) == hipSuccess && hipMalloc(&dc, meta.bytes) == hipSuccess) {
+      hipError_t e = hipHostGetDevicePointer(&src, pinned, 0);
+      if (e == hipSuccess) e = hipMalloc(&dc, meta.bytes);
+      if (e == hipSuccess) {
         hipLaunchKernelGGL(pull_table, dim3(kPullBlocks), dim3(256), 0, upload_stream,
                            (const unsigned char *)src, (unsigned char *)dc, meta.bytes);
-        if (hipGetLastError() == hipSuccess && hipStreamSynchronize(upload_stream) == hipSuccess)
-          d = esk_table_wrap(&meta, dc);
-        if (!d) (void)hipFree(dc);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(upload_stream);
       }
+      if (e != hipSuccess) err = esk_hip_fail(e, "server: re-admitting a table");
+      else if (!(d = esk_table_wrap(&meta, dc))) err = EINSPLINE_ENOMEM;
+      if (!d && dc) (void)hipFree(dc);
     } else {
       d = from_double ? create_multi_UBspline_3d_s_hip_store((const multi_UBspline_3d_d *)host, storage)
                       : create_multi_UBspline_3d_s_hip((multi_UBspline_3d_s *)host);
+      if (!d) err = EINSPLINE_ENOMEM;   // the create call's message says what failed
       // keep a storage-format host copy so later re-admissions skip conversion
       if (d && hipHostMalloc(&new_pinned, d->bytes, hipHostMallocDefault) == hipSuccess) {
         if (hipMemcpy(new_pinned, d->coefs, d->bytes, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -429,6 +447,11 @@ void einspline_server::upload_loop() {
         u.meta = *d;
         u.meta.coefs = nullptr;
       }
+    } else {
+      // Requests for this table cannot be served: fail the server rather
+      // than retry the upload for ever (waiters and submitters get the error).
+      set_error(err);
+      cv_done.notify_all();
     }
     cv_work.notify_all();
   }
@@ -439,11 +462,13 @@ void einspline_server::poll_completions() {
     if (!s.busy) continue;
     hipError_t e = hipEventQuery(s.done);
     if (e == hipErrorNotReady) continue;
-    if (e != hipSuccess) worker_error = esk_hip_fail(e, "server batch");
+    const bool bad = e != hipSuccess;
+    if (bad) set_error(esk_hip_fail(e, "server batch"));
     const auto now = clk::now();
     for (Req *r : s.reqs) {
       r->latency = std::chrono::duration<double>(now - r->t0).count();
       r->done = true;
+      r->failed |= bad;
       tables[r->table].inflight--;
       st.completed++;
       st.evals += (double)r->npts * tables[r->table].nsplines;
@@ -565,7 +590,10 @@ bool einspline_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
   if (e == hipSuccess) e = hipGraphLaunch(exec, s.stream);
   if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
   lk.lock();
-  if (e != hipSuccess) worker_error = esk_hip_fail(e, "server launch");
+  if (e != hipSuccess) {
+    set_error(esk_hip_fail(e, "server launch"));
+    for (Req *r : s.reqs) r->failed = true;
+  }
   return true;
 }
 
@@ -577,10 +605,13 @@ void einspline_server::run() {
     cv_done.notify_all();
     bool any_busy = false;
     for (Slot &s : slots) any_busy |= s.busy;
-    if (stop && pending.empty() && !any_busy) break;
+    // After a failure no batch is formed: the batches in flight finish and
+    // the requests still queued are released by einspline_server_destroy.
+    if (stop && !any_busy && (pending.empty() || worker_error)) break;
     Slot *free_slot = nullptr;
-    for (Slot &s : slots)
-      if (!s.busy) { free_slot = &s; break; }
+    if (!worker_error)
+      for (Slot &s : slots)
+        if (!s.busy) { free_slot = &s; break; }
     bool launched = false;
     if (free_slot && !pending.empty()) launched = launch_batch(*free_slot, lk);
     if (!launched) {
@@ -682,7 +713,7 @@ static int submit_impl(einspline_server *s, int tid, const double *pos, int npts
     }
     if (s->worker_error) {
       delete r;
-      return s->worker_error;
+      return s->error_for_caller();
     }
     r->table = tid;
     r->id = s->next_id++;
@@ -710,12 +741,13 @@ static int finish(einspline_server *s, uint64_t id, double *lat, bool block) {
   if (!r->done) {
     if (!block) return 0;
     s->cv_done.wait(lk, [&] { return r->done || s->worker_error; });
-    if (!r->done) return s->worker_error;
+    if (!r->done) return s->error_for_caller();
   }
   if (lat) *lat = r->latency;
+  const bool failed = r->failed;
   s->live.erase(it);
   delete r;
-  return 1;
+  return failed ? s->error_for_caller() : 1;
 }
 
 extern "C" int einspline_server_test(einspline_server *s, uint64_t t, double *lat) {
@@ -739,7 +771,7 @@ extern "C" int einspline_server_drain(einspline_server *s) {
       if (sl.busy) return false;
     return true;
   });
-  return s->worker_error;
+  return s->worker_error ? s->error_for_caller() : EINSPLINE_OK;
 }
 
 extern "C" int einspline_server_get_stats(einspline_server *s, einspline_server_stats *out) {

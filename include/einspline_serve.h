@@ -76,9 +76,13 @@ int einspline_server_submit(einspline_server *srv, int table_id, const double *p
                             int npts, void *vals_d, void *grads_d, void *hess_d,
                             uint64_t *ticket);
 
-/* 1 if the request finished (latency in seconds stored), 0 if pending. */
+/* 1 if the request finished (latency in seconds stored), 0 if pending.  A
+ * request whose batch or table upload failed returns the failure's code once
+ * collected; after any failure the server forms no more batches and every
+ * call that submits or waits returns that code. */
 int einspline_server_test(einspline_server *srv, uint64_t ticket, double *latency_s);
-/* Block until the request finished. */
+/* Block until the request finished (EINSPLINE_OK), or a failure code as for
+ * einspline_server_test. */
 int einspline_server_wait(einspline_server *srv, uint64_t ticket, double *latency_s);
 /* Block until every submitted request finished. */
 int einspline_server_drain(einspline_server *srv);
