@@ -30,8 +30,9 @@
 //
 // Results leave through LDS: lane (g, r) holds rows 4g .. 4g+3 of its
 // orbitals, and writes them to the wavefront's staging area at their places
-// in the output layout; the wavefront then copies the slab's v, gradient and
-// Hessian blocks out as contiguous runs (16-byte stores where aligned).
+// in the output layout; once two slabs are staged the wavefront copies their
+// v, gradient and Hessian blocks out as contiguous runs (16-byte stores
+// where aligned).
 #pragma once
 #include <type_traits>
 #include "eval_kernel.hpp"
@@ -80,9 +81,11 @@ template <> struct MfmaRows<true> {
   static constexpr uint32_t OZ = 1u << 6 | 1u << 12 | 1u << 16 | 2u << 18;
 };
 
-// Floats of one wavefront's staging area: v, gradient and Hessian blocks of a
-// 128-orbital slab in output layout (the full Hessian, the larger case).
-constexpr int kMfmaStage = 128 * (1 + 3 + 9);
+// Floats of one wavefront's staging area: v, gradient and Hessian blocks of
+// two 128-orbital slabs in output layout (the full Hessian, the larger case).
+// Four wavefronts take 53 KB: three blocks fill a CU's 160 KB, as many as
+// the registers allow.
+constexpr int kMfmaStage = 256 * (1 + 3 + 9);
 
 // The wavefront's point (blocks of four wavefronts, XCD runs and the
 // traversal order as in eval3d): its cell and basis values (value, first and
@@ -150,44 +153,18 @@ __device__ __forceinline__ void mfma_copy_out(int lane, const float *src, float 
   for (int i = done + lane; i < n; i += 64) dst[i] = src[i];
 }
 
-// Copy a full 128-orbital slab's staged v, gradient and Hessian blocks
-// (128 (4 + HS) floats, contiguous in the staging area) out with a fixed
-// sequence of store instructions, every one executed by all 64 lanes and
-// none behind a branch: the compiler then knows how many stores follow the
-// next slab's loads and waits for those loads only, not for the stores to
-// complete.  16-byte (and 8-byte) stores if the destinations are 16-byte
-// aligned, else one float per lane per instruction.
-template <int HS>
-__device__ __forceinline__ void mfma_copy_full(int lane, const float *stage, float *ov, float *og, float *oh,
-                                               bool vec, bool nt) {
+// Streaming copy of N floats (a multiple of 128) from the staging area to
+// 16-byte aligned dst: 16-byte pieces, consecutive lanes filling whole
+// 128-byte lines, every instruction but a last half-wavefront one executed by
+// all 64 lanes.
+template <int N>
+__device__ __forceinline__ void mfma_stream_out(int lane, const float *src, float *dst) {
   typedef float v4 __attribute__((ext_vector_type(4)));
-  typedef float v2 __attribute__((ext_vector_type(2)));
-  const float *sh = stage + 512;
-  if (vec && nt) {
-    const uint32_t l8 = 8u * lane, l16 = 16u * lane;
-    st_stream(*(const v2 *)(stage + 2 * lane), ov, l8);
-    st_stream(*(const v4 *)(stage + 128 + 4 * lane), og, l16);
-    st_stream(*(const v2 *)(stage + 384 + 2 * lane), og, 1024u + l8);
+  static_assert(N % 128 == 0, "whole lines for every group of eight lanes");
 #pragma unroll
-    for (int k = 0; k < HS * 128 / 256; ++k) st_stream(*(const v4 *)(sh + 256 * k + 4 * lane), oh, 1024u * k + l16);
-    if constexpr (HS * 128 % 256 != 0) st_stream(*(const v2 *)(sh + HS * 128 - 128 + 2 * lane), oh, 4u * (HS * 128 - 128) + l8);
-  } else if (vec) {
-    *(v2 *)(ov + 2 * lane) = *(const v2 *)(stage + 2 * lane);                   // v: 128 floats
-    *(v4 *)(og + 4 * lane) = *(const v4 *)(stage + 128 + 4 * lane);             // gradient: 384
-    *(v2 *)(og + 256 + 2 * lane) = *(const v2 *)(stage + 384 + 2 * lane);
-#pragma unroll
-    for (int k = 0; k < HS * 128 / 256; ++k)                                     // Hessian: 128 HS
-      *(v4 *)(oh + 256 * k + 4 * lane) = *(const v4 *)(sh + 256 * k + 4 * lane);
-    if constexpr (HS * 128 % 256 != 0)
-      *(v2 *)(oh + HS * 128 - 128 + 2 * lane) = *(const v2 *)(sh + HS * 128 - 128 + 2 * lane);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) ov[64 * k + lane] = stage[64 * k + lane];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) og[64 * k + lane] = stage[128 + 64 * k + lane];
-#pragma unroll
-    for (int k = 0; k < 2 * HS; ++k) oh[64 * k + lane] = sh[64 * k + lane];
-  }
+  for (int k = 0; k < N / 256; ++k) st_stream(*(const v4 *)(src + 256 * k + 4 * lane), dst, 1024u * k + 16u * lane);
+  if constexpr (N % 256 != 0)
+    if (lane < 32) st_stream(*(const v4 *)(src + N - 128 + 4 * lane), dst, 4u * (N - 128) + 16u * lane);
 }
 
 // Register budget: three wavefronts per SIMD (at most 168 VGPRs; the
@@ -198,9 +175,9 @@ __device__ __forceinline__ void mfma_copy_full(int lane, const float *stage, flo
 // fixed store sequence.  With the store variants chosen at run time inside
 // the slab loop, the compiler's count of outstanding memory operations at
 // the loop head is the most conservative over the variants' paths, and it
-// then waits for the previous slab's stores (vector-memory counts include
-// stores) before the slab's last rows: a store round trip per slab.  With
-// one path the waits are exact and the stores stay in flight.
+// then waits for the previous slabs' stores (vector-memory counts include
+// stores) before the next rows.  With one path the waits are exact and the
+// stores stay in flight.
 template <class S, bool PACKED, bool STREAM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 eval3d_mfma(const Params P) {
@@ -276,19 +253,20 @@ eval3d_mfma(const Params P) {
   // 8r + t (tile t) of the slab.
   float sc[4];
   mfma_scales<Rows>(P, g, oscale, sc);
-  // The staging area holds the slab in output layout: v block (128 floats),
-  // gradient block (384), Hessian block (128 HS).  Lanes g = 0 hold v and
-  // the gradient of their eight orbitals (8 and 24 consecutive floats),
-  // lanes g > 0 Hessian entries 4 (g - 1) + e at stride HS per orbital; each
-  // group's writes are its base address plus constant offsets.
-  float *const vg_base = stage + 8 * r;                          // v of orbital 8r
-  float *const gr_base = stage + 128 + 24 * r;                   // gradient of orbital 8r
-  float *const h_base = stage + 512 + Rows::HS * 8 * r + 4 * (g - 1);
+  // The staging area holds two slabs in output layout: v block (256 floats),
+  // gradient block (768), Hessian block (256 HS); slab `half` (0 or 1) of a
+  // pair fills the second half of each block or the first.  Lanes g = 0
+  // hold v and the gradient of their eight orbitals (8 and 24 consecutive
+  // floats), lanes g > 0 Hessian entries 4 (g - 1) + e at stride HS per
+  // orbital; each group's writes are its base address plus constant offsets.
+  constexpr int HS = Rows::HS;
+  float *const vg_base = stage + 8 * r;                            // v of orbital 8r
+  float *const gr_base = stage + 256 + 24 * r;                     // gradient of orbital 8r
+  float *const h_base = stage + 1024 + HS * 8 * r + 4 * (g - 1);
 
   const S *cell = (const S *)P.coefs + (long)ix * P.xs + (long)iy * P.ys + (long)iz * P.zs;
   const int row_reals = (int)P.zs;
   const int end = min(row_reals, P.nreal);            // slabs start below this
-  const bool nt = P.store == STORE_STAGED_NT;
   float *ov = (float *)P.out_v + (long)p * P.ld_v;
   float *og = (float *)P.out_g + (long)p * P.ld_g;
   float *oh = (float *)P.out_h + (long)p * P.ld_h;
@@ -340,8 +318,10 @@ eval3d_mfma(const Params P) {
   // Software pipeline over slabs: the first half's registers are refilled
   // with the next slab's rows once its products are issued, the second's
   // once the slab is staged in LDS, so the whole next slab is in flight
-  // while this one is stored.  The last
-  // slab is peeled off (no refill), which keeps every load unconditional.
+  // while this one is stored.  The last slab takes no refill, which keeps
+  // every load unconditional.  Scheduling barriers keep the refills where
+  // they are written: left to itself the scheduler moved them behind the
+  // other half's products.
   uint4 R0[8], R1[8];
   // all slabs of the point, or only the wavefront's slab (orbital slabs outermost)
   const int s_first = slab < 0 ? 0 : slab * 128;
@@ -349,65 +329,93 @@ eval3d_mfma(const Params P) {
   if (s_first >= s_last) return;
   load_half(R0, s_first, 0);
   load_half(R1, s_first, 1);
-  // Slabs before the last are whole (s0 + 128 < s_last <= nreal) and take
-  // the fixed store sequence.
-  const bool aligned = (((uintptr_t)ov | (uintptr_t)og | (uintptr_t)oh) & 15) == 0;
-  auto slab_step = [&](int s0, bool more, bool full) __attribute__((always_inline)) {
+  auto slab_step = [&](int s0, bool more, int half) __attribute__((always_inline)) {
     f4 acc[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
     mma_half(R0, acc, 0);
+    __builtin_amdgcn_sched_barrier(0);
     if (more) load_half(R0, s0 + 128, 0);
+    __builtin_amdgcn_sched_barrier(0);
     mma_half(R1, acc, 1);
     // stage the slab (orbitals past the valid ones are staged but not
     // copied out)
     typedef float v4 __attribute__((ext_vector_type(4)));
+    float *const vb = vg_base + 128 * half, *const gb = gr_base + 384 * half, *const hb = h_base + 128 * HS * half;
     if (g == 0) {
-      *(v4 *)vg_base = v4{acc[0][0], acc[1][0], acc[2][0], acc[3][0]} * sc[0];
-      *(v4 *)(vg_base + 4) = v4{acc[4][0], acc[5][0], acc[6][0], acc[7][0]} * sc[0];
+      *(v4 *)vb = v4{acc[0][0], acc[1][0], acc[2][0], acc[3][0]} * sc[0];
+      *(v4 *)(vb + 4) = v4{acc[4][0], acc[5][0], acc[6][0], acc[7][0]} * sc[0];
 #pragma unroll
       for (int t = 0; t < 8; t += 4) {
         const v4 a = {acc[t][1] * sc[1], acc[t][2] * sc[2], acc[t][3] * sc[3], acc[t + 1][1] * sc[1]};
         const v4 b = {acc[t + 1][2] * sc[2], acc[t + 1][3] * sc[3], acc[t + 2][1] * sc[1], acc[t + 2][2] * sc[2]};
         const v4 c = {acc[t + 2][3] * sc[3], acc[t + 3][1] * sc[1], acc[t + 3][2] * sc[2], acc[t + 3][3] * sc[3]};
-        *(v4 *)(gr_base + 3 * t) = a;
-        *(v4 *)(gr_base + 3 * t + 4) = b;
-        *(v4 *)(gr_base + 3 * t + 8) = c;
+        *(v4 *)(gb + 3 * t) = a;
+        *(v4 *)(gb + 3 * t + 4) = b;
+        *(v4 *)(gb + 3 * t + 8) = c;
       }
     } else {
 #pragma unroll
       for (int t = 0; t < 8; ++t)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          if (4 * g + e < Rows::NQ) h_base[Rows::HS * t + e] = acc[t][e] * sc[e];
+          if (4 * g + e < Rows::NQ) hb[HS * t + e] = acc[t][e] * sc[e];
     }
     wave_sync();
     // the accumulators are staged: their registers take the next rows
+    __builtin_amdgcn_sched_barrier(0);
     if (more) load_half(R1, s0 + 128, 1);
-    if (full) {
-      if constexpr (STREAM)
-        mfma_copy_full<Rows::HS>(lane, stage, ov + s0, og + 3 * s0, oh + Rows::HS * s0, true, true);
-      else
-        mfma_copy_full<Rows::HS>(lane, stage, ov + s0, og + 3 * s0, oh + Rows::HS * s0, aligned, nt);
-    } else {
-      const int m = min(128, P.nreal - s0);           // valid orbitals of the slab
-      mfma_copy_out(lane, stage, ov + s0, m);
-      mfma_copy_out(lane, stage + 128, og + 3 * s0, 3 * m);
-      mfma_copy_out(lane, stage + 512, oh + Rows::HS * s0, Rows::HS * m);
-    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  // Copy out the staged orbitals [p0, p0 + m) (one or two slabs).  STREAM
+  // launches write whole slabs with a fixed sequence of streaming 16-byte
+  // stores (NS slabs: compile-time); everything else goes through
+  // mfma_copy_out.
+  auto store_whole = [&](int p0, auto ns) __attribute__((always_inline)) {
+    constexpr int NS = decltype(ns)::value;
+    mfma_stream_out<128 * NS>(lane, stage, ov + p0);
+    mfma_stream_out<384 * NS>(lane, stage + 256, og + 3 * p0);
+    mfma_stream_out<128 * HS * NS>(lane, stage + 1024, oh + HS * p0);
     wave_sync();                                      // reads done before the next slab's writes
   };
-  // The first slab is peeled off the loop too, so that on entry to it, as
-  // on every later pass, the next slab's loads are followed by exactly the
-  // slab's stores.  The last slab takes the fixed (streaming) sequence too
-  // when it is whole: with cached stores there, a quarter of a 512-orbital
-  // point's results evicted table rows from L2.
-  int s0 = s_first;
-  if (s0 + 128 < s_last) {
-    slab_step(s0, true, true);
-    for (s0 += 128; s0 + 128 < s_last; s0 += 128) slab_step(s0, true, true);
+  auto store_some = [&](int p0, int m) __attribute__((always_inline)) {
+    mfma_copy_out(lane, stage, ov + p0, m);
+    mfma_copy_out(lane, stage + 256, og + 3 * p0, 3 * m);
+    mfma_copy_out(lane, stage + 1024, oh + HS * p0, HS * m);
+    wave_sync();
+  };
+  // Slabs go in pairs, stored together once both are staged: a point's
+  // results leave in half as many bursts, each of whole 16-byte stores.
+  // Every pair followed by more slabs is whole (its end is below s_last,
+  // which is at most nreal) and takes one fixed store sequence.  The first
+  // such pair is peeled off the loop, so that the loop is entered, as it is
+  // repeated, with a pair's stores behind the next rows: with one path and
+  // one entry state the compiler's wait counts at the loop head are exact
+  // and the stores stay in flight while the next pair's first rows arrive.
+  auto pair_step = [&](int p0) __attribute__((always_inline)) {
+    slab_step(p0, true, 0);
+    slab_step(p0 + 128, true, 1);
+    if constexpr (STREAM) store_whole(p0, std::integral_constant<int, 2>{});
+    else store_some(p0, 256);
+  };
+  const int nsl = (s_last - s_first + 127) >> 7;
+  int k = 0;
+  if (k + 2 < nsl) {
+    pair_step(s_first);
+    for (k = 2; k + 2 < nsl; k += 2) pair_step(s_first + 128 * k);
   }
-  slab_step(s0, false, s0 + 128 <= P.nreal);
+  const int p0 = s_first + 128 * k;
+  const int m = s_last - p0;                          // valid orbitals left: 1 .. 256
+  if (nsl - k == 2) {
+    slab_step(p0, true, 0);
+    slab_step(p0 + 128, false, 1);
+    if (STREAM && m == 256) store_whole(p0, std::integral_constant<int, 2>{});
+    else store_some(p0, m);
+  } else {
+    slab_step(p0, false, 0);
+    if (STREAM && m == 128) store_whole(p0, std::integral_constant<int, 1>{});
+    else store_some(p0, m);
+  }
 }
 
 // ---------------------------------------------------------------------------
