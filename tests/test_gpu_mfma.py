@@ -78,7 +78,11 @@ def full_to_packed(H):
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("N", [5, 40, 128, 130, 200, 512])
+# 128-orbital slabs go in pairs: N covers one partial slab (5, 40), one whole
+# (128), a whole pair with or without a partial one (130, 200, 300), two pairs
+# (512), the pair loop once (768) and twice (898, 1000, partial last pair);
+# N % 4 != 0 (5, 130, 898) takes the plain-store variant.
+@pytest.mark.parametrize("N", [5, 40, 128, 130, 200, 300, 512, 768, 898, 1000])
 @pytest.mark.parametrize("packed", [False, True])
 def test_matrix_kernel_matches_oracle_and_vector_kernel(storage, N, packed, rng):
     M = (9, 10, 11)
@@ -97,6 +101,22 @@ def test_matrix_kernel_matches_oracle_and_vector_kernel(storage, N, packed, rng)
     if not packed:
         # the full Hessian is written symmetric
         np.testing.assert_array_equal(mH, np.swapaxes(mH, 2, 3))
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+@pytest.mark.parametrize("N", [512, 1000])
+def test_matrix_kernel_store_policies_agree(storage, N, rng):
+    """Streaming stores (aligned outputs) and the cached and direct policies
+    write the same bits, whole pairs and a partial last pair alike."""
+    h, tab, o = build(storage, (9, 10, 11), N, rng)
+    pts = rng.random((257, 3))
+    res = []
+    for policy in (0, 1, 2):
+        tab.set_store_policy(policy)
+        res.append(eval_vgh(tab, pts, ea.KERNEL_MATRIX))
+    for other in res[1:]:
+        for x, y in zip(res[0], other):
+            np.testing.assert_array_equal(x, y)
 
 
 @pytest.mark.parametrize("storage", STORAGES)
