@@ -93,7 +93,7 @@ constexpr int kMfmaStage = 256 * (1 + 3 + 9);
 __device__ __forceinline__ bool mfma_point(const Params &P, int &p, int &slab, int &ix, int &iy, int &iz,
                                            float (&wx)[3][4], float (&wy)[3][4], float (&wz)[3][4]) {
   slab = -1;                          // all slabs of the point
-  if (P.slabs > 1) {                  // orbital slabs outermost: one slab per wavefront
+  if (P.slabs > 1) {                  // orbital slabs outermost: one 256-orbital slab per wavefront
     if (!slab_wave(P, blockIdx.x, blockDim.x >> 6, threadIdx.x >> 6, p, slab)) return false;
     p = uniform(p);                   // wave-uniform by construction; said so to the compiler
     slab = uniform(slab);
@@ -323,9 +323,10 @@ eval3d_mfma(const Params P) {
   // they are written: left to itself the scheduler moved them behind the
   // other half's products.
   uint4 R0[8], R1[8];
-  // all slabs of the point, or only the wavefront's slab (orbital slabs outermost)
-  const int s_first = slab < 0 ? 0 : slab * 128;
-  const int s_last = slab < 0 ? end : min(end, s_first + 128);
+  // all slabs of the point, or only the wavefront's 256 orbitals (orbital
+  // slabs outermost: one pair of 128-orbital blocks per wavefront)
+  const int s_first = slab < 0 ? 0 : slab * 256;
+  const int s_last = slab < 0 ? end : min(end, s_first + 256);
   if (s_first >= s_last) return;
   load_half(R0, s_first, 0);
   load_half(R1, s_first, 1);

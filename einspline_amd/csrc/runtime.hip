@@ -818,7 +818,10 @@ static int run_strided(const multi_UBspline_3d_hip *t, int mode, const double *p
   if (((const TableExt *)t)->point_order & EINSPLINE_ORDER_SLABS) {
     // orbital slabs outermost: a wavefront per (point, slab), slab-major
     if (esk::use_mfma(L)) {
-      const int w = t->storage == EINSPLINE_STORE_F32 ? 64 : 128;    // the matrix kernels' slab width
+      // the matrix kernels' slab width: 64 orbitals for FP32; for 16-bit
+      // tables one pair of 128-orbital blocks, which a wavefront stages and
+      // stores in one burst
+      const int w = t->storage == EINSPLINE_STORE_F32 ? 64 : 256;
       const int end = std::min(t->row_reals, P.nreal);
       P.slabs = (end + w - 1) / w;
     } else if (P.chunks % 64 == 0 && L.block % 64 == 0) {

@@ -287,7 +287,7 @@ enum { EINSPLINE_ORDER_GIVEN = 0, EINSPLINE_ORDER_LOCALITY = 1 };
 /* Either order may add EINSPLINE_ORDER_SLABS (order | EINSPLINE_ORDER_SLABS):
  * orbital slabs outermost.  When a point's row spans several wavefronts'
  * widths (vector kernel: more than 64 lanes per point; matrix kernels: more
- * than 128 orbitals, or 64 for FP32 tables), all points evaluate the first
+ * than 256 orbitals for 16-bit tables, or 64 for FP32 tables), all points evaluate the first
  * slab of orbitals, then the second, and so on, so the rows of one slab are
  * what the caches hold at a time.  Results are identical.  Applies to the
  * strided einspline_eval_*_batch calls; elsewhere it is ignored. */

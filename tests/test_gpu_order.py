@@ -168,13 +168,16 @@ def test_autotune_on_clustered_caller_points(rng):
                                                    (ea.STORE_BF16, ea.KERNEL_VECTOR, (8, 64, 1), 512),
                                                    (ea.STORE_BF16, ea.KERNEL_MATRIX, (16, 256, 0), 300),
                                                    (ea.STORE_F16, ea.KERNEL_MATRIX, (16, 256, 1), 300),
+                                                   (ea.STORE_F16, ea.KERNEL_MATRIX, (16, 256, 0), 512),
+                                                   (ea.STORE_BF16, ea.KERNEL_MATRIX, (16, 256, 0), 1000),
                                                    (ea.STORE_F32, ea.KERNEL_MATRIX, (16, 256, 1), 300)])
 def test_orbital_slab_orders_are_bitwise_identical(storage, kernel, tile, N, rng):
     """EINSPLINE_ORDER_SLABS (orbital slabs outermost) with either point order
     gives the bits of the plain given order.  Rows span 2-5 wavefront widths
-    (vector kernel: 128-192 lanes per point; matrix kernels: 300 orbitals,
-    the last slab partial); 5000 points, so the locality sort runs and the
-    per-XCD split of the points is uneven."""
+    (vector kernel: 128-192 lanes per point; 16-bit matrix kernel: 256-orbital
+    slabs, 300 and 1000 orbitals ending in a partial slab, 512 in two whole
+    ones; FP32 matrix kernel: 64-orbital slabs); 5000 points, so the locality
+    sort runs and the per-XCD split of the points is uneven."""
     P = 5000
     h = ea.HostSpline("d", (11, 9, 12), (0, 0, 0), (1, 1, 1), N)
     h.set_block(0, rng.uniform(-1, 1, (N, 11, 9, 12)))
