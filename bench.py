@@ -536,7 +536,9 @@ def precision(args, res):
                     "_given_order: the same kernels on the points as given, where every stencil "
                     "row comes from memory and the element size decides; in the locality order "
                     "most rows are cache hits and the 52 B/eval of FP32 results, the same for "
-                    "every storage type, dominate the traffic")
+                    "every storage type, dominate the traffic; both launches are then bound by "
+                    "each CU's memory path, where the result stores wait behind the row loads' "
+                    "L2 misses")
     rows["workload"] = "periodic 96^3, 512 orbitals, vgh at 131072 points; FP64 build"
     res.setdefault("extra", {})["precision"] = rows
 
